@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: fixed-source S_N solve on one MI355X.
+
+Workload (one "step"): the seeded random-block problem -- 64^3 cells of 1 cm,
+3 groups (down-scatter only), 128 directions (GC 4 polar x 4 azimuthal per
+octant), theta-weighted diamond difference (theta = 0.9), FP64, central unit
+source in group 0 -- solved from zero by multigroup Gauss-Seidel with
+scattering source iteration to a 1e-6 relative residual, exactly the
+reference's solveFixedSource loop.  Inputs are resident in HBM before timing.
+
+Prints one JSON line.  ``value`` is the end-to-end grind time
+ns / (cell * direction * group-sweep) of the timed solves (lower is better).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# MI355X published peaks.
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# Algorithmic flops per cell-direction update as written in the reference
+# (divide = 1 flop): TWDD3D 110, DD3D 30 (TWDiamondDifference3D.h:43-86,
+# DiamondDifference3D.h:35-43).
+FLOPS_PER_UNIT = {1: 110, 0: 30}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(problem, seconds: float, threads: int):
+    """CPU restatement of the reference's OpenMP parallel-in-angle sweep, timed on
+    a bounded sample of the same workload (whole sweeps of group 0)."""
+    import numpy as np
+
+    from oracle import binding as ob
+
+    rng = np.random.default_rng(0)
+    q = rng.uniform(0.0, 1.0, problem.N) / (4 * np.pi)
+    ob.sweep(problem, 0, q, nthreads=threads)  # warm-up (first-touch, page-in)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        ob.sweep(problem, 0, q, nthreads=threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 50:
+            break
+    per = el / n
+    units = problem.N * problem.noct * problem.dirs
+    return {"value": per * 1e9 / units, "unit": "ns/(cell*dir*grp)", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} full sweeps of group 0 of the same 64^3 x 128-direction problem "
+                      f"({per * 1e3:.1f} ms/sweep, {el:.1f} s), OpenMP parallel-in-angle "
+                      f"with per-thread partials, {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--exact", action="store_true", help="reference expression order")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the sweep on a single MI355X "
+              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from doma_amd import workloads
+    from doma_amd.capi import FP64, Context
+
+    p = workloads.random_blocks(args.n, 3, 4, 4)
+    tol = 1e-6
+    log(f"bench: {p.name} N={p.N} directions={p.noct * p.dirs} groups={p.groups}")
+    ctx = Context(p, precision=FP64)
+    tuning, tune_ms = ctx.autotune(0)
+    if args.exact:
+        ctx.set_tuning(brick=tuning["brick"], threads=tuning["threads"], exact_arith=1)
+        tuning = ctx.get_tuning()
+    log(f"bench: autotuned {tuning} ({tune_ms:.3f} ms/sweep)")
+
+    for i in range(args.warmup):
+        _, st = ctx.solve_fixed_source(tol, 1000, tol, 100, want_flux=False)
+        log(f"bench: warmup {i} {st['elapsed_ms']:.1f} ms, SI {st['si_iterations']}")
+    times, sweeps, sweep_ms = [], 0, 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        _, st = ctx.solve_fixed_source(tol, 1000, tol, 100, want_flux=False)
+        times.append(time.perf_counter() - ts)
+        sweeps += st["sweeps"]
+        sweep_ms += st["sweep_ms"]
+        log(f"bench: step {i} {times[-1] * 1e3:.1f} ms, SI {st['si_iterations']}, "
+            f"MG {st['mg_iterations']}")
+    total = time.perf_counter() - t0
+    assert st["success"], "solve did not converge"
+    units_sweep = ctx.units_per_sweep()
+    ms_per_step = total / args.steps * 1e3
+    grind_ns = total * 1e9 / (sweeps * units_sweep)
+
+    k_ms, w_ms, launches = ctx.profile_sweeps(0, 5)
+    flops_sweep = units_sweep * FLOPS_PER_UNIT[p.scheme]
+    achieved = flops_sweep / (k_ms * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / "sweep_hbm_traffic.json"
+    if tfile.exists():
+        tj = json.loads(tfile.read_text())
+        traffic = tj.get(f"{p.name}", {}).get("bytes_per_launch")
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": traffic,
+                "kernel": "brick_sweep_kernel", "avg_launch_us": round(k_ms * 1e3 / launches, 2),
+                "launches_per_sweep": launches, "flops_per_unit": FLOPS_PER_UNIT[p.scheme]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"bench: CPU baseline with {threads} threads")
+        cpu = cpu_baseline(p, args.cpu_seconds, threads)
+
+    out = {
+        "metric": "grind ns/(cell*dir*grp) + time-to-1e-6 residual, learned vs plain SI",
+        "value": round(grind_ns, 5),
+        "unit": "ns/(cell*dir*grp)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-block media, deterministic)",
+        "config": {"workload": p.name, "cells": p.N, "groups": p.groups,
+                   "directions": p.noct * p.dirs, "scheme": "tw_dd (theta=0.9)",
+                   "tolerance": tol, "precision": "fp64", "tuning": tuning},
+        "time_to_solution_ms": round(ms_per_step, 3),
+        "sweeps_per_solve": sweeps // args.steps,
+        "sweep_kernel_grind_ns": round(k_ms * 1e6 / units_sweep, 5),
+        "sweep_wall_grind_ns": round(w_ms * 1e6 / units_sweep, 5),
+        "learned_vs_plain": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

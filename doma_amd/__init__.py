@@ -1,0 +1,16 @@
+"""doma_amd -- MI355X-native discrete-ordinates sweep and scattering source
+iteration (DOMA fixed-source path) on hand-written HIP kernels for gfx950.
+
+The compute path is the native library ``libdoma_amd.so`` (C ABI in
+``include/doma_c.h``); this package only binds it.  There is no CPU fallback:
+every entry point raises when the library or a GPU is unavailable.
+"""
+from .capi import (  # noqa: F401
+    FP32_SWEEP, FP64, SCHEME_DD, SCHEME_TW_DD, Context, DomaError, MediaSpec, Problem,
+    gc_quadrature, gl_quadrature, lib, media_generate,
+)
+
+__all__ = [
+    "Context", "DomaError", "MediaSpec", "Problem", "gc_quadrature", "gl_quadrature", "lib",
+    "media_generate", "FP64", "FP32_SWEEP", "SCHEME_DD", "SCHEME_TW_DD",
+]

@@ -1,0 +1,333 @@
+"""ctypes binding of the doma_amd C ABI (include/doma_c.h).
+
+The native library ``libdoma_amd.so`` is built in tree by ``doma_amd.build``.
+Importing this module never falls back to a CPU path: if the library is missing
+or cannot be loaded, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libdoma_amd.so"
+
+DOMA_OK = 0
+DOMA_ERR_INVALID = 1
+DOMA_ERR_DEVICE = 2
+DOMA_ERR_STATE = 3
+
+SCHEME_DD = 0
+SCHEME_TW_DD = 1
+FP64 = 0
+FP32_SWEEP = 1
+MAX_GROUPS = 64
+
+SCATTER_DOWN1 = 0
+SCATTER_FULL = 1
+SOURCE_CENTRAL = 0
+SOURCE_BLOCKS = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("lx", _dp), ("ly", _dp), ("lz", _dp),
+        ("groups", C.c_int32), ("scheme", C.c_int32), ("precision", C.c_int32),
+        ("dirs", C.c_int32),
+        ("mu", _dp), ("eta", _dp), ("xi", _dp), ("w", _dp),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("brick", C.c_int32), ("threads", C.c_int32), ("exact_arith", C.c_int32),
+                ("schedule", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("success", C.c_int32), ("mg_iterations", C.c_int32), ("mg_residual", C.c_double),
+        ("sweeps", C.c_int32), ("si_iterations", C.c_int32 * MAX_GROUPS),
+        ("si_residual", C.c_double * MAX_GROUPS), ("elapsed_ms", C.c_double),
+        ("sweep_ms", C.c_double),
+    ]
+
+
+class MediaSpec(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("block", C.c_int32), ("groups", C.c_int32), ("sigt_log", C.c_int32),
+        ("sigt_lo", C.c_double), ("sigt_hi", C.c_double), ("c_lo", C.c_double),
+        ("c_hi", C.c_double), ("scatter_rule", C.c_int32), ("source_rule", C.c_int32),
+        ("src_lo", C.c_int32), ("src_hi", C.c_int32), ("source_fraction", C.c_double),
+        ("cell_width", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+_LIB = None
+
+# (name, restype, argtypes) of every exported entry point.
+SIGNATURES = [
+    ("doma_last_error", C.c_char_p, []),
+    ("doma_version", C.c_char_p, []),
+    ("doma_create", C.c_int, [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
+    ("doma_destroy", C.c_int, [C.c_void_p]),
+    ("doma_set_materials", C.c_int, [C.c_void_p, _ip, C.c_int32, _dp, _dp, _dp, _u8p]),
+    ("doma_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("doma_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("doma_sweep", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
+    ("doma_sweep_device", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("doma_source_iteration", C.c_int,
+     [C.c_void_p, C.c_int32, _dp, C.c_double, C.c_int32, _dp, _ip, _dp, _ip]),
+    ("doma_solve_fixed_source", C.c_int,
+     [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32, _dp, C.POINTER(Stats)]),
+    ("doma_get_flux", C.c_int, [C.c_void_p, _dp]),
+    ("doma_time_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _ip]),
+    ("doma_profile_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _ip]),
+    ("doma_units_per_sweep", C.c_int64, [C.c_void_p]),
+    ("doma_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning), _dp]),
+    ("doma_gc_quadrature", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp]),
+    ("doma_gl_quadrature", C.c_int, [C.c_int32, _dp, _dp]),
+    ("doma_media_num_materials", C.c_int32, [C.POINTER(MediaSpec)]),
+    ("doma_media_generate", C.c_int, [C.POINTER(MediaSpec), _ip, _dp, _dp, _dp, _u8p]),
+]
+
+
+def lib() -> C.CDLL:
+    """Load libdoma_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m doma_amd.build`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+class DomaError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = lib().doma_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def _check(code: int, where: str):
+    if code != DOMA_OK:
+        raise DomaError(code, where)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gc_quadrature(n_c: int, n_l: int, dims: int):
+    """Directions/weights of GCAngularQuadrature(n_c, n_l, dims) as [noct, per] arrays."""
+    noct = 8 if dims == 3 else 4
+    per = (n_c * n_l) // 4
+    arrs = [np.zeros(noct * max(per, 1)) for _ in range(4)]
+    r = lib().doma_gc_quadrature(n_c, n_l, dims, *[_d(a) for a in arrs])
+    if r < 0:
+        raise ValueError(f"GC quadrature ({n_c}, {n_l}, {dims}) has a zero direction component")
+    return tuple(a.reshape(noct, per) for a in arrs)
+
+
+def gl_quadrature(n: int):
+    """GLAngularQuadrature(n) nodes (descending) and weights, as [2, n/2] half ranges."""
+    mu, w = np.zeros(n), np.zeros(n)
+    r = lib().doma_gl_quadrature(n, _d(mu), _d(w))
+    if r < 0:
+        raise ValueError("GL quadrature needs an even order >= 2")
+    return mu.reshape(2, n // 2), w.reshape(2, n // 2)
+
+
+@dataclass
+class Problem:
+    """A fixed-source problem in the C-ABI layout (numpy, host)."""
+    dims: int
+    lx: np.ndarray
+    ly: np.ndarray
+    lz: np.ndarray
+    groups: int
+    scheme: int
+    mat: np.ndarray          # int32 [nz, ny, nx] or flat
+    sigt: np.ndarray         # [M, G]
+    sigs: np.ndarray         # [M, G, G] dest-major
+    src: np.ndarray          # [M, G]
+    has_src: np.ndarray      # uint8 [M]
+    mu: np.ndarray           # [noct, dirs]
+    eta: np.ndarray
+    xi: np.ndarray
+    w: np.ndarray
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nx(self):
+        return len(self.lx)
+
+    @property
+    def ny(self):
+        return len(self.ly)
+
+    @property
+    def nz(self):
+        return len(self.lz)
+
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def dirs(self):
+        return self.mu.shape[1]
+
+    @property
+    def noct(self):
+        return self.mu.shape[0]
+
+    @property
+    def num_mats(self):
+        return self.sigt.shape[0]
+
+    def normalized(self) -> "Problem":
+        """Contiguous arrays with the exact dtypes the C ABI expects."""
+        return Problem(
+            self.dims, _f64(self.lx), _f64(self.ly), _f64(self.lz), int(self.groups),
+            int(self.scheme), np.ascontiguousarray(self.mat, dtype=np.int32).reshape(-1),
+            _f64(self.sigt).reshape(-1, self.groups),
+            _f64(self.sigs).reshape(-1, self.groups, self.groups),
+            _f64(self.src).reshape(-1, self.groups),
+            np.ascontiguousarray(self.has_src, dtype=np.uint8).reshape(-1),
+            _f64(self.mu), _f64(self.eta), _f64(self.xi), _f64(self.w), self.name, dict(self.meta))
+
+
+class Context:
+    """Owns a doma_ctx: device-resident problem, sweep and iteration drivers."""
+
+    def __init__(self, problem: Problem, precision: int = FP64, tuning: dict | None = None):
+        p = problem.normalized()
+        self.problem = p
+        self._keep = p  # arrays must outlive the create call
+        d = ProblemDesc(p.dims, p.nx, p.ny, p.nz, _d(p.lx), _d(p.ly), _d(p.lz), p.groups,
+                        p.scheme, precision, p.dirs, _d(p.mu), _d(p.eta), _d(p.xi), _d(p.w))
+        h = C.c_void_p()
+        _check(lib().doma_create(C.byref(d), C.byref(h)), "doma_create")
+        self.h = h
+        _check(lib().doma_set_materials(h, p.mat.ctypes.data_as(_ip), p.num_mats, _d(p.sigt),
+                                        _d(p.sigs), _d(p.src), p.has_src.ctypes.data_as(_u8p)),
+               "doma_set_materials")
+        if tuning:
+            self.set_tuning(**tuning)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().doma_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, brick=0, threads=0, exact_arith=0, schedule=0):
+        t = Tuning(brick, threads, exact_arith, schedule)
+        _check(lib().doma_set_tuning(self.h, C.byref(t)), "doma_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().doma_get_tuning(self.h, C.byref(t)), "doma_get_tuning")
+        return {"brick": t.brick, "threads": t.threads, "exact_arith": t.exact_arith,
+                "schedule": t.schedule}
+
+    def sweep(self, g: int, q: np.ndarray) -> np.ndarray:
+        q = _f64(q).reshape(-1)
+        out = np.zeros(self.problem.N)
+        _check(lib().doma_sweep(self.h, g, _d(q), _d(out)), "doma_sweep")
+        return out
+
+    def source_iteration(self, g: int, qmg: np.ndarray, tol: float, max_it: int):
+        qmg = _f64(qmg).reshape(-1)
+        phi = np.zeros(self.problem.N)
+        its, res, conv = C.c_int32(), C.c_double(), C.c_int32()
+        _check(lib().doma_source_iteration(self.h, g, _d(qmg), tol, max_it, _d(phi), C.byref(its),
+                                           C.byref(res), C.byref(conv)), "doma_source_iteration")
+        return phi, its.value, res.value, bool(conv.value)
+
+    def solve_fixed_source(self, fs_tol: float, fs_max_it: int, mg_tol: float, mg_max_it: int,
+                           want_flux: bool = True):
+        phi = np.zeros(self.problem.groups * self.problem.N) if want_flux else None
+        st = Stats()
+        _check(lib().doma_solve_fixed_source(self.h, fs_tol, fs_max_it, mg_tol, mg_max_it,
+                                             _d(phi) if want_flux else None, C.byref(st)),
+               "doma_solve_fixed_source")
+        stats = {
+            "success": bool(st.success), "mg_iterations": st.mg_iterations,
+            "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+            "si_iterations": list(st.si_iterations[: self.problem.groups]),
+            "si_residual": list(st.si_residual[: self.problem.groups]),
+            "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms,
+        }
+        if want_flux:
+            phi = phi.reshape(self.problem.groups, self.problem.N)
+        return phi, stats
+
+    def get_flux(self) -> np.ndarray:
+        phi = np.zeros(self.problem.groups * self.problem.N)
+        _check(lib().doma_get_flux(self.h, _d(phi)), "doma_get_flux")
+        return phi.reshape(self.problem.groups, self.problem.N)
+
+    def time_sweeps(self, g: int, n: int):
+        ms, launches = C.c_double(), C.c_int32()
+        _check(lib().doma_time_sweeps(self.h, g, n, C.byref(ms), C.byref(launches)),
+               "doma_time_sweeps")
+        return ms.value, launches.value
+
+    def profile_sweeps(self, g: int, n: int):
+        """(kernel ms per sweep, wall ms per sweep, launches per sweep) from HIP events
+        recorded around every launch on the library's stream."""
+        k, w, L = C.c_double(), C.c_double(), C.c_int32()
+        _check(lib().doma_profile_sweeps(self.h, g, n, C.byref(k), C.byref(w), C.byref(L)),
+               "doma_profile_sweeps")
+        return k.value, w.value, L.value
+
+    def units_per_sweep(self) -> int:
+        return int(lib().doma_units_per_sweep(self.h))
+
+    def autotune(self, g: int = 0):
+        t, ms = Tuning(), C.c_double()
+        _check(lib().doma_autotune(self.h, g, C.byref(t), C.byref(ms)), "doma_autotune")
+        return {"brick": t.brick, "threads": t.threads, "exact_arith": t.exact_arith}, ms.value
+
+
+def media_generate(spec: MediaSpec):
+    """Seeded block media: (mat[n^3], sigt[M,G], sigs[M,G,G], src[M,G], has_src[M])."""
+    M = lib().doma_media_num_materials(C.byref(spec))
+    if M < 0:
+        raise ValueError("invalid media spec")
+    n, G = spec.n, spec.groups
+    mat = np.zeros(n * n * n, dtype=np.int32)
+    sigt, sigs, src = np.zeros((M, G)), np.zeros((M, G, G)), np.zeros((M, G))
+    has = np.zeros(M, dtype=np.uint8)
+    _check(lib().doma_media_generate(C.byref(spec), mat.ctypes.data_as(_ip), _d(sigt), _d(sigs),
+                                     _d(src), has.ctypes.data_as(_u8p)), "doma_media_generate")
+    return mat, sigt, sigs, src, has

@@ -1,0 +1,930 @@
+// capi.hip -- the extern "C" entry points declared in include/doma_c.h: context
+// and device memory management, the brick schedule of the sweep, and the source
+// iteration / multigroup Gauss-Seidel drivers (TransportSolver3D.cpp:262-563 and
+// the 2D/1D counterparts) with all per-cell state resident in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "internal.h"
+
+namespace
+{
+thread_local std::string g_last_error;
+
+template <typename T>
+int
+dev_alloc(T ** p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0)
+    count = 1;
+  DOMA_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count));
+  return DOMA_OK;
+}
+
+template <typename T>
+void
+dev_free(T *& p)
+{
+  if (p)
+    (void)hipFree(reinterpret_cast<void *>(p));
+  p = nullptr;
+}
+
+int
+next_pow2(int v)
+{
+  int p = 1;
+  while (p < v)
+    p <<= 1;
+  return p;
+}
+
+// Autotuner cache: (dims, n, G-independent shape, dirs, precision, scheme) -> tuning.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int, int, int>, doma_tuning> g_tune_cache;
+
+std::tuple<int, int, int, int, int, int, int>
+tune_key(const doma_ctx * c)
+{
+  return std::make_tuple(c->dims, c->nx, c->ny, c->nz, c->dirs, c->precision, c->scheme);
+}
+} // namespace
+
+namespace doma
+{
+int
+fail(int code, const std::string & msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+int
+hip_fail(hipError_t e, const char * what)
+{
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return DOMA_ERR_DEVICE;
+}
+
+IterArgs
+iter_args(doma_ctx * c, int g)
+{
+  IterArgs a{};
+  a.dims = c->dims;
+  a.N = c->N;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.G = c->G;
+  a.g = g;
+  a.ndirsets = c->ndirsets;
+  a.mat = c->d_mat;
+  a.sigt = c->d_sigt;
+  a.sigs = c->d_sigs;
+  a.src = c->d_src;
+  a.has_src = c->d_has_src;
+  a.lx = c->d_lx;
+  a.ly = c->d_ly;
+  a.lz = c->d_lz;
+  a.sig = c->d_sig;
+  a.q = c->d_q;
+  a.qmg = c->d_qmg;
+  a.phi = c->d_phi;
+  a.prev = c->d_prev;
+  a.part = c->d_part;
+  a.dphi = c->d_dphi;
+  a.red = c->d_red;
+  a.res = c->d_res;
+  return a;
+}
+
+void
+free_sweep(doma_ctx * c)
+{
+  dev_free(c->d_bricks);
+  dev_free(c->d_plane_cells);
+  dev_free(c->d_plane_off);
+  dev_free(c->d_fx);
+  dev_free(c->d_fy);
+  dev_free(c->d_fz);
+  dev_free(c->d_part);
+}
+
+// Brick schedule, hyperplane tables and face/partial buffers for the current
+// tuning.  1D uses one thread per direction and no bricks.
+int
+configure_sweep(doma_ctx * c)
+{
+  free_sweep(c);
+  if (c->dims == 1)
+  {
+    c->ndirsets = 2 * c->dirs;
+    c->launches_per_sweep = 1;
+    return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
+  }
+  const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
+  int B = c->tuning.brick;
+  if (B == 0)
+    B = c->dims == 3 ? (minext >= 48 ? 8 : 4) : (minext >= 64 ? 16 : 8);
+  int T = c->tuning.threads ? c->tuning.threads : 256;
+  c->nchunks = (c->dirs + 63) / 64;
+  c->chunk = (c->dirs + c->nchunks - 1) / c->nchunks;
+  c->dl = std::max(8, next_pow2(c->chunk));
+  if (!brick_variant_exists(c->dims, B, c->dl, T))
+    return fail(DOMA_ERR_INVALID, "no sweep kernel for brick " + std::to_string(B) + ", " +
+                                      std::to_string(T) + " threads, " + std::to_string(c->dl) +
+                                      " direction lanes");
+  c->brick = B;
+  c->threads = T;
+  c->exact = c->tuning.exact_arith ? 1 : 0;
+  c->ndirsets = c->noct * c->nchunks;
+
+  const int ntx = (c->nx + B - 1) / B, nty = (c->ny + B - 1) / B;
+  const int ntz = c->dims == 3 ? (c->nz + B - 1) / B : 1;
+  c->nxp = ntx * B;
+  c->nyp = nty * B;
+  c->nzp = c->dims == 3 ? ntz * B : 1;
+
+  // Bricks sorted by diagonal bi+bj+bk (oriented coordinates).
+  const int ndiag = ntx + nty + ntz - 2;
+  std::vector<int16_t> bricks;
+  c->diag_off.assign(ndiag + 1, 0);
+  for (int s = 0; s < ndiag; ++s)
+  {
+    c->diag_off[s] = (int)(bricks.size() / 3);
+    for (int bk = 0; bk < ntz; ++bk)
+      for (int bj = 0; bj < nty; ++bj)
+      {
+        const int bi = s - bj - bk;
+        if (bi < 0 || bi >= ntx)
+          continue;
+        bricks.push_back((int16_t)bi);
+        bricks.push_back((int16_t)bj);
+        bricks.push_back((int16_t)bk);
+      }
+  }
+  c->diag_off[ndiag] = (int)(bricks.size() / 3);
+  c->launches_per_sweep = ndiag;
+
+  // Brick-local cells ordered by hyperplane li+lj+lk.
+  const int bz = c->dims == 3 ? B : 1;
+  const int nplanes = c->dims == 3 ? 3 * B - 2 : 2 * B - 1;
+  std::vector<uint32_t> cells;
+  std::vector<int32_t> off(nplanes + 1, 0);
+  for (int p = 0; p < nplanes; ++p)
+  {
+    off[p] = (int32_t)cells.size();
+    for (int lk = 0; lk < bz; ++lk)
+      for (int lj = 0; lj < B; ++lj)
+      {
+        const int li = p - lj - lk;
+        if (li < 0 || li >= B)
+          continue;
+        cells.push_back((uint32_t)li | ((uint32_t)lj << 8) | ((uint32_t)lk << 16));
+      }
+  }
+  off[nplanes] = (int32_t)cells.size();
+
+  int rc;
+  if ((rc = dev_alloc(&c->d_bricks, bricks.size())) ||
+      (rc = dev_alloc(&c->d_plane_cells, cells.size())) ||
+      (rc = dev_alloc(&c->d_plane_off, off.size())))
+    return rc;
+  DOMA_HIP(hipMemcpy(c->d_bricks, bricks.data(), bricks.size() * sizeof(int16_t),
+                     hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_plane_cells, cells.data(), cells.size() * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_plane_off, off.data(), off.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+
+  const size_t rs = c->precision == DOMA_FP32_SWEEP ? sizeof(float) : sizeof(double);
+  size_t nfx, nfy, nfz;
+  if (c->dims == 3)
+  {
+    nfx = (size_t)c->ndirsets * c->nyp * c->nzp * c->dl;
+    nfy = (size_t)c->ndirsets * c->nxp * c->nzp * c->dl;
+    nfz = (size_t)c->ndirsets * c->nxp * c->nyp * c->dl;
+  }
+  else
+  {
+    nfx = (size_t)c->ndirsets * c->nyp * c->dl;
+    nfy = (size_t)c->ndirsets * c->nxp * c->dl;
+    nfz = 1;
+  }
+  c->face_bytes = (nfx + nfy + nfz) * rs;
+  DOMA_HIP(hipMalloc(&c->d_fx, nfx * rs));
+  DOMA_HIP(hipMalloc(&c->d_fy, nfy * rs));
+  DOMA_HIP(hipMalloc(&c->d_fz, nfz * rs));
+  return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
+}
+
+// One sweep of group g over every octant: q = c->d_q, sigma_t = c->d_sig,
+// partials -> c->d_part.
+int
+run_sweep_with(doma_ctx * c, const double * q)
+{
+  if (c->dims == 1)
+  {
+    Sweep1DArgs a{};
+    a.nx = c->nx;
+    a.dirs = c->dirs;
+    a.dirtab = c->d_dirtab;
+    a.lx = c->d_lx;
+    a.q = q;
+    a.sig = c->d_sig;
+    a.part = c->d_part;
+    return launch_sweep1d(a, c->precision, c->stream);
+  }
+  BrickArgs a{};
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.N = c->N;
+  a.dirs = c->dirs;
+  a.chunk = c->chunk;
+  a.nchunks = c->nchunks;
+  a.ndirsets = c->ndirsets;
+  a.dirtab = c->d_dirtab;
+  a.lx = c->d_lx;
+  a.ly = c->d_ly;
+  a.lz = c->d_lz;
+  a.rlx = c->d_rlx;
+  a.rly = c->d_rly;
+  a.rlz = c->d_rlz;
+  a.q = q;
+  a.sig = c->d_sig;
+  a.part = c->d_part;
+  a.fx = c->d_fx;
+  a.fy = c->d_fy;
+  a.fz = c->d_fz;
+  a.nxp = c->nxp;
+  a.nyp = c->nyp;
+  a.nzp = c->nzp;
+  a.bricks = c->d_bricks;
+  a.plane_cells = c->d_plane_cells;
+  a.plane_off = c->d_plane_off;
+  const int ndiag = (int)c->diag_off.size() - 1;
+  for (int s = 0; s < ndiag; ++s)
+  {
+    a.diag_begin = c->diag_off[s];
+    a.diag_count = c->diag_off[s + 1] - c->diag_off[s];
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[2 * s], c->stream));
+    int rc = launch_brick_sweep(a, c->dims, c->scheme, c->precision, c->exact, c->brick, c->dl,
+                                c->threads, c->stream);
+    if (rc)
+      return rc;
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[2 * s + 1], c->stream));
+  }
+  return DOMA_OK;
+}
+
+int
+run_sweep(doma_ctx * c, int)
+{
+  return run_sweep_with(c, c->d_q);
+}
+} // namespace doma
+
+using namespace doma;
+
+namespace
+{
+int
+check_ctx(const doma_ctx * c, bool need_materials)
+{
+  if (!c)
+    return fail(DOMA_ERR_INVALID, "null context");
+  if (need_materials && !c->materials)
+    return fail(DOMA_ERR_STATE, "materials have not been set (doma_set_materials)");
+  return DOMA_OK;
+}
+
+// Scattering source iteration of group g from the state prepared by the caller
+// (TransportSolver3D.cpp:458-497): sweep, residual, source update, until
+// it == max_it or the residual is <= tol.
+int
+si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, int * conv,
+        double * sweep_ms, int * sweeps)
+{
+  IterArgs a = iter_args(c, g);
+  int it = 0;
+  double r = 0.0;
+  do
+  {
+    DOMA_HIP(hipEventRecord(c->ev0, c->stream));
+    int rc = run_sweep(c, g);
+    if (rc)
+      return rc;
+    DOMA_HIP(hipEventRecord(c->ev1, c->stream));
+    if ((rc = launch_finish(a, c->red_blocks, c->stream)) ||
+        (rc = launch_residual_final(a, c->red_blocks, c->stream)))
+      return rc;
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    DOMA_HIP(hipStreamSynchronize(c->stream));
+    DOMA_HIP(hipGetLastError());
+    float ms = 0.f;
+    DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (sweep_ms)
+      *sweep_ms += ms;
+    if (sweeps)
+      ++*sweeps;
+    r = c->h_res[0];
+    ++it;
+  } while (it < max_it && tol < r);
+  if (its)
+    *its = it;
+  if (res)
+    *res = r;
+  if (conv)
+    *conv = it < max_it ? 1 : 0;
+  return DOMA_OK;
+}
+
+double
+now_ms()
+{
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+} // namespace
+
+extern "C" const char *
+doma_last_error(void)
+{
+  return g_last_error.c_str();
+}
+
+extern "C" const char *
+doma_version(void)
+{
+  return "doma_amd 0.1 (gfx950)";
+}
+
+extern "C" int
+doma_create(const doma_problem_desc * d, doma_ctx ** out)
+{
+  if (!d || !out)
+    return fail(DOMA_ERR_INVALID, "doma_create: null argument");
+  *out = nullptr;
+  if (d->dims < 1 || d->dims > 3)
+    return fail(DOMA_ERR_INVALID, "dims must be 1, 2 or 3");
+  const int ny = d->dims > 1 ? d->ny : 1;
+  const int nz = d->dims > 2 ? d->nz : 1;
+  if (d->nx < 1 || ny < 1 || nz < 1)
+    return fail(DOMA_ERR_INVALID, "cell counts must be positive");
+  if ((double)d->nx * ny * nz > 2.0e9)
+    return fail(DOMA_ERR_INVALID, "mesh too large (cell index must fit in int32)");
+  if (d->groups < 1 || d->groups > DOMA_MAX_GROUPS)
+    return fail(DOMA_ERR_INVALID, "groups must be in [1, 64]");
+  if (d->scheme != DOMA_SCHEME_DD && d->scheme != DOMA_SCHEME_TW_DD)
+    return fail(DOMA_ERR_INVALID, "unknown scheme");
+  if (d->dims == 1 && d->scheme == DOMA_SCHEME_TW_DD)
+    return fail(DOMA_ERR_INVALID,
+                "Theta-Weighted Diamond Differences are currently not supported in slab geometries!");
+  if (d->precision != DOMA_FP64 && d->precision != DOMA_FP32_SWEEP)
+    return fail(DOMA_ERR_INVALID, "unknown precision mode");
+  if (d->dirs < 1 || d->dirs > 4096)
+    return fail(DOMA_ERR_INVALID, "directions per octant must be in [1, 4096]");
+  if (!d->lx || (d->dims > 1 && !d->ly) || (d->dims > 2 && !d->lz) || !d->mu || !d->w ||
+      (d->dims > 1 && (!d->eta || !d->xi)))
+    return fail(DOMA_ERR_INVALID, "null geometry or quadrature array");
+  const int noct = d->dims == 3 ? 8 : (d->dims == 2 ? 4 : 2);
+  for (int t = 0; t < noct * d->dirs; ++t)
+  {
+    if (!(d->w[t] >= 0.0) || !std::isfinite(d->mu[t]) || d->mu[t] == 0.0)
+      return fail(DOMA_ERR_INVALID, "invalid direction or weight");
+    if (d->dims > 1 && (d->eta[t] == 0.0 || !std::isfinite(d->eta[t]) || !std::isfinite(d->xi[t])))
+      return fail(DOMA_ERR_INVALID, "invalid direction");
+  }
+  auto chk_w = [](const double * l, int n) {
+    for (int i = 0; i < n; ++i)
+      if (!(l[i] > 0.0) || !std::isfinite(l[i]))
+        return false;
+    return true;
+  };
+  if (!chk_w(d->lx, d->nx) || (d->dims > 1 && !chk_w(d->ly, ny)) ||
+      (d->dims > 2 && !chk_w(d->lz, nz)))
+    return fail(DOMA_ERR_INVALID, "cell widths must be positive and finite");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(DOMA_ERR_DEVICE, "no HIP device available: the doma_amd sweep runs only on the GPU");
+
+  doma_ctx * c = new (std::nothrow) doma_ctx();
+  if (!c)
+    return fail(DOMA_ERR_INVALID, "out of host memory");
+  c->dims = d->dims;
+  c->nx = d->nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->N = d->nx * ny * nz;
+  c->G = d->groups;
+  c->scheme = d->scheme;
+  c->precision = d->precision;
+  c->dirs = d->dirs;
+  c->noct = noct;
+  int rc = DOMA_OK;
+  auto cleanup = [&](int code) {
+    doma_destroy(c);
+    return code;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "hipGetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "stream/event creation failed"));
+
+  std::vector<double> lyv(ny, 1.0), lzv(nz, 1.0);
+  if (d->dims > 1)
+    lyv.assign(d->ly, d->ly + ny);
+  if (d->dims > 2)
+    lzv.assign(d->lz, d->lz + nz);
+  std::vector<double> lxv(d->lx, d->lx + d->nx);
+  std::vector<double> rlx(d->nx), rly(ny), rlz(nz);
+  for (int i = 0; i < d->nx; ++i)
+    rlx[i] = 1.0 / lxv[i];
+  for (int j = 0; j < ny; ++j)
+    rly[j] = 1.0 / lyv[j];
+  for (int k = 0; k < nz; ++k)
+    rlz[k] = 1.0 / lzv[k];
+  std::vector<double> dirtab((size_t)noct * d->dirs * 4);
+  for (int t = 0; t < noct * d->dirs; ++t)
+  {
+    dirtab[4 * t + 0] = std::fabs(d->mu[t]);
+    dirtab[4 * t + 1] = d->dims > 1 ? std::fabs(d->eta[t]) : 0.0;
+    dirtab[4 * t + 2] = d->dims > 1 ? std::fabs(d->xi[t]) : 0.0;
+    dirtab[4 * t + 3] = d->w[t];
+  }
+  // 1D keeps the signed cosine (DiamondDifference1D takes |mu| itself).
+  if (d->dims == 1)
+    for (int t = 0; t < noct * d->dirs; ++t)
+      dirtab[4 * t + 0] = std::fabs(d->mu[t]);
+
+  const size_t N = (size_t)c->N;
+  if ((rc = dev_alloc(&c->d_lx, d->nx)) || (rc = dev_alloc(&c->d_ly, ny)) ||
+      (rc = dev_alloc(&c->d_lz, nz)) || (rc = dev_alloc(&c->d_rlx, d->nx)) ||
+      (rc = dev_alloc(&c->d_rly, ny)) || (rc = dev_alloc(&c->d_rlz, nz)) ||
+      (rc = dev_alloc(&c->d_dirtab, dirtab.size())) || (rc = dev_alloc(&c->d_sig, N)) ||
+      (rc = dev_alloc(&c->d_q, N)) || (rc = dev_alloc(&c->d_qmg, N)) ||
+      (rc = dev_alloc(&c->d_dphi, N)) || (rc = dev_alloc(&c->d_phi, N * c->G)) ||
+      (rc = dev_alloc(&c->d_prev, N * c->G)))
+    return cleanup(rc);
+  c->red_blocks = reduction_blocks(c->N);
+  if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 4)))
+    return cleanup(rc);
+  if (hipHostMalloc(reinterpret_cast<void **>(&c->h_res), 4 * sizeof(double)) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
+  if (hipMemcpy(c->d_lx, lxv.data(), lxv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_ly, lyv.data(), lyv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_lz, lzv.data(), lzv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rlx, rlx.data(), rlx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rly, rly.data(), rly.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rlz, rlz.data(), rlz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_dirtab, dirtab.data(), dirtab.size() * 8, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "upload of geometry/quadrature failed"));
+  if ((rc = launch_fill(c->d_phi, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_prev, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_q, N, 0.0, c->stream)))
+    return cleanup(rc);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(tune_key(c));
+    if (it != g_tune_cache.end())
+    {
+      c->tuning = it->second;
+      c->autotuned = 1;
+    }
+  }
+  if ((rc = configure_sweep(c)))
+    return cleanup(rc);
+  if (hipStreamSynchronize(c->stream) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "initialisation failed"));
+  *out = c;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_destroy(doma_ctx * c)
+{
+  if (!c)
+    return DOMA_OK;
+  if (c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  free_sweep(c);
+  dev_free(c->d_lx);
+  dev_free(c->d_ly);
+  dev_free(c->d_lz);
+  dev_free(c->d_rlx);
+  dev_free(c->d_rly);
+  dev_free(c->d_rlz);
+  dev_free(c->d_dirtab);
+  dev_free(c->d_mat);
+  dev_free(c->d_sigt);
+  dev_free(c->d_sigs);
+  dev_free(c->d_src);
+  dev_free(c->d_has_src);
+  dev_free(c->d_sig);
+  dev_free(c->d_q);
+  dev_free(c->d_qmg);
+  dev_free(c->d_dphi);
+  dev_free(c->d_phi);
+  dev_free(c->d_prev);
+  dev_free(c->d_red);
+  dev_free(c->d_res);
+  if (c->h_res)
+    (void)hipHostFree(c->h_res);
+  if (c->ev0)
+    (void)hipEventDestroy(c->ev0);
+  if (c->ev1)
+    (void)hipEventDestroy(c->ev1);
+  for (auto e : c->prof_events)
+    (void)hipEventDestroy(e);
+  if (c->stream)
+    (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_materials(doma_ctx * c, const int32_t * mat, int32_t M, const double * sigt,
+                   const double * sigs, const double * src, const uint8_t * has_src)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (!mat || !sigt || !sigs || !src || !has_src || M < 1)
+    return fail(DOMA_ERR_INVALID, "doma_set_materials: null array or no materials");
+  for (int i = 0; i < c->N; ++i)
+    if (mat[i] < 0 || mat[i] >= M)
+      return fail(DOMA_ERR_INVALID,
+                  "Block " + std::to_string(mat[i]) + " (defined on the mesh) has no material properties!");
+  const size_t G = c->G;
+  for (size_t e = 0; e < (size_t)M * G; ++e)
+    if (!std::isfinite(sigt[e]) || !std::isfinite(src[e]))
+      return fail(DOMA_ERR_INVALID, "non-finite cross section or source");
+  for (size_t e = 0; e < (size_t)M * G * G; ++e)
+    if (!std::isfinite(sigs[e]))
+      return fail(DOMA_ERR_INVALID, "non-finite scattering cross section");
+  if (c->M != M)
+  {
+    dev_free(c->d_sigt);
+    dev_free(c->d_sigs);
+    dev_free(c->d_src);
+    dev_free(c->d_has_src);
+    c->M = 0;
+    if ((rc = dev_alloc(&c->d_sigt, (size_t)M * G)) || (rc = dev_alloc(&c->d_sigs, (size_t)M * G * G)) ||
+        (rc = dev_alloc(&c->d_src, (size_t)M * G)) || (rc = dev_alloc(&c->d_has_src, (size_t)M)))
+      return rc;
+    c->M = M;
+  }
+  if (!c->d_mat && (rc = dev_alloc(&c->d_mat, (size_t)c->N)))
+    return rc;
+  DOMA_HIP(hipMemcpy(c->d_mat, mat, sizeof(int32_t) * c->N, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_sigt, sigt, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_sigs, sigs, sizeof(double) * M * G * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_src, src, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_has_src, has_src, (size_t)M, hipMemcpyHostToDevice));
+  c->materials = true;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_tuning(doma_ctx * c, const doma_tuning * t)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (!t)
+    return fail(DOMA_ERR_INVALID, "null tuning");
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  doma_tuning old = c->tuning;
+  c->tuning = *t;
+  if ((rc = configure_sweep(c)))
+  {
+    c->tuning = old;
+    int rc2 = configure_sweep(c);
+    (void)rc2;
+    return rc;
+  }
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_get_tuning(const doma_ctx * c, doma_tuning * t)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (!t)
+    return fail(DOMA_ERR_INVALID, "null tuning");
+  t->brick = c->brick;
+  t->threads = c->threads;
+  t->exact_arith = c->exact;
+  t->schedule = 0;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_sweep_device(doma_ctx * c, int32_t g, const double * d_q, double * d_dphi)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (g < 0 || g >= c->G || !d_q || !d_dphi)
+    return fail(DOMA_ERR_INVALID, "doma_sweep: bad group or null array");
+  IterArgs a = iter_args(c, g);
+  if ((rc = launch_gather_sig(a, c->stream)) || (rc = run_sweep_with(c, d_q)))
+    return rc;
+  a.dphi = d_dphi;
+  if ((rc = launch_reduce_parts(a, c->stream)))
+    return rc;
+  DOMA_HIP(hipGetLastError());
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_sweep(doma_ctx * c, int32_t g, const double * q, double * dphi)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!q || !dphi)
+    return fail(DOMA_ERR_INVALID, "doma_sweep: null array");
+  DOMA_HIP(hipMemcpyAsync(c->d_q, q, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+  if ((rc = doma_sweep_device(c, g, c->d_q, c->d_dphi)))
+    return rc;
+  DOMA_HIP(hipMemcpyAsync(dphi, c->d_dphi, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_source_iteration(doma_ctx * c, int32_t g, const double * qmg, double tol, int32_t max_it,
+                      double * phi_g, int32_t * iterations, double * residual, int32_t * converged)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (g < 0 || g >= c->G || !qmg)
+    return fail(DOMA_ERR_INVALID, "doma_source_iteration: bad group or null source");
+  if (!(tol >= 0.0) || max_it < 1)
+    return fail(DOMA_ERR_INVALID, "doma_source_iteration: tolerance must be >= 0 and max_it >= 1");
+  DOMA_HIP(hipMemcpyAsync(c->d_dphi, qmg, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+  IterArgs a = iter_args(c, g);
+  if ((rc = launch_set_q(a, c->d_dphi, c->stream)))
+    return rc;
+  int its = 0, conv = 0;
+  double r = 0.0;
+  if ((rc = si_loop(c, g, tol, max_it, &its, &r, &conv, nullptr, nullptr)))
+    return rc;
+  if (phi_g)
+  {
+    DOMA_HIP(hipMemcpyAsync(phi_g, c->d_phi + (size_t)g * c->N, sizeof(double) * c->N,
+                            hipMemcpyDeviceToHost, c->stream));
+    DOMA_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (iterations)
+    *iterations = its;
+  if (residual)
+    *residual = r;
+  if (converged)
+    *converged = conv;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_tol,
+                        int32_t mg_max_it, double * phi, doma_stats * st)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
+    return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
+  const double t0 = now_ms();
+  doma_stats s{};
+  const size_t N = (size_t)c->N;
+  // initializeSolve (TransportSolver3D.cpp:499-529): phi = 0.
+  if ((rc = launch_fill(c->d_phi, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_prev, N * c->G, 0.0, c->stream)))
+    return rc;
+  int mg_it = 0, ok = 1;
+  double r = 0.0;
+  do
+  {
+    for (int g = 0; g < c->G; ++g)
+    {
+      IterArgs a = iter_args(c, g);
+      if ((rc = launch_mg_source(a, c->stream)))
+        return rc;
+      int its = 0, conv = 0;
+      double res = 0.0;
+      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps)))
+        return rc;
+      s.si_iterations[g] = its;
+      s.si_residual[g] = res;
+      if (!conv)
+      {
+        ok = 0;
+        break;
+      }
+    }
+    if (!ok)
+      break;
+    IterArgs a = iter_args(c, 0);
+    if ((rc = launch_mg_residual(a, c->red_blocks, c->stream)))
+      return rc;
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    DOMA_HIP(hipStreamSynchronize(c->stream));
+    r = c->h_res[0];
+    ++mg_it;
+  } while (mg_it < mg_max_it && mg_tol < r && c->G > 1);
+  s.success = ok && mg_it < mg_max_it;
+  s.mg_iterations = mg_it;
+  s.mg_residual = r;
+  if (phi)
+  {
+    DOMA_HIP(hipMemcpyAsync(phi, c->d_phi, sizeof(double) * N * c->G, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipGetLastError());
+  s.elapsed_ms = now_ms() - t0;
+  if (st)
+    *st = s;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_get_flux(doma_ctx * c, double * phi)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (!phi)
+    return fail(DOMA_ERR_INVALID, "null output");
+  DOMA_HIP(hipMemcpyAsync(phi, c->d_phi, sizeof(double) * (size_t)c->N * c->G,
+                          hipMemcpyDeviceToHost, c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  return DOMA_OK;
+}
+
+extern "C" int64_t
+doma_units_per_sweep(const doma_ctx * c)
+{
+  if (!c)
+    return -1;
+  return (int64_t)c->N * c->noct * c->dirs;
+}
+
+extern "C" int
+doma_time_sweeps(doma_ctx * c, int32_t g, int32_t n, double * ms_per_sweep, int32_t * launches)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (g < 0 || g >= c->G || n < 1)
+    return fail(DOMA_ERR_INVALID, "doma_time_sweeps: bad group or count");
+  IterArgs a = iter_args(c, g);
+  if ((rc = launch_gather_sig(a, c->stream)) || (rc = run_sweep(c, g)))
+    return rc;  // warm-up
+  DOMA_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < n; ++i)
+    if ((rc = run_sweep(c, g)))
+      return rc;
+  DOMA_HIP(hipEventRecord(c->ev1, c->stream));
+  DOMA_HIP(hipEventSynchronize(c->ev1));
+  DOMA_HIP(hipGetLastError());
+  float ms = 0.f;
+  DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (ms_per_sweep)
+    *ms_per_sweep = ms / n;
+  if (launches)
+    *launches = c->launches_per_sweep;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_profile_sweeps(doma_ctx * c, int32_t g, int32_t n, double * kernel_ms, double * wall_ms,
+                    int32_t * launches)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (g < 0 || g >= c->G || n < 1)
+    return fail(DOMA_ERR_INVALID, "doma_profile_sweeps: bad group or count");
+  const int L = c->launches_per_sweep;
+  const bool per_launch = c->dims != 1;
+  if (per_launch)
+    while ((int)c->prof_events.size() < 2 * L)
+    {
+      hipEvent_t e;
+      DOMA_HIP(hipEventCreate(&e));
+      c->prof_events.push_back(e);
+    }
+  IterArgs a = iter_args(c, g);
+  if ((rc = launch_gather_sig(a, c->stream)) || (rc = run_sweep(c, g)))
+    return rc;  // warm-up
+  double ksum = 0.0, wsum = 0.0;
+  for (int it = 0; it < n; ++it)
+  {
+    c->profiling = per_launch;
+    DOMA_HIP(hipEventRecord(c->ev0, c->stream));
+    rc = run_sweep(c, g);
+    c->profiling = false;
+    if (rc)
+      return rc;
+    DOMA_HIP(hipEventRecord(c->ev1, c->stream));
+    DOMA_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    wsum += ms;
+    if (per_launch)
+      for (int s = 0; s < L; ++s)
+      {
+        DOMA_HIP(hipEventElapsedTime(&ms, c->prof_events[2 * s], c->prof_events[2 * s + 1]));
+        ksum += ms;
+      }
+    else
+      ksum += ms;
+  }
+  DOMA_HIP(hipGetLastError());
+  if (kernel_ms)
+    *kernel_ms = ksum / n;
+  if (wall_ms)
+    *wall_ms = wsum / n;
+  if (launches)
+    *launches = L;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (g < 0 || g >= c->G)
+    return fail(DOMA_ERR_INVALID, "doma_autotune: bad group");
+  std::vector<doma_tuning> cands;
+  if (c->dims == 3)
+    for (int b : {4, 8})
+      for (int t : {128, 256, 512})
+        cands.push_back(doma_tuning{b, t, 0, 0});
+  else if (c->dims == 2)
+    for (int b : {8, 16})
+      cands.push_back(doma_tuning{b, 256, 0, 0});
+  else
+    cands.push_back(doma_tuning{0, 0, 0, 0});
+  const doma_tuning keep = c->tuning;
+  doma_tuning win = keep;
+  double win_ms = 1e300;
+  for (const auto & t : cands)
+  {
+    c->tuning = t;
+    if (configure_sweep(c) != DOMA_OK)
+      continue;  // shape not available for this direction count
+    double ms = 0.0;
+    if ((rc = doma_time_sweeps(c, g, 3, &ms, nullptr)))
+    {
+      c->tuning = keep;
+      configure_sweep(c);
+      return rc;
+    }
+    if (ms < win_ms)
+    {
+      win_ms = ms;
+      win = t;
+      win.exact_arith = keep.exact_arith;
+    }
+  }
+  c->tuning = win;
+  if ((rc = configure_sweep(c)))
+    return rc;
+  c->autotuned = 1;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[tune_key(c)] = win;
+  }
+  if (best)
+    doma_get_tuning(c, best);
+  if (best_ms)
+    *best_ms = win_ms;
+  return DOMA_OK;
+}

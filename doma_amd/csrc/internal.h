@@ -1,0 +1,179 @@
+// internal.h -- host-side context and helpers shared by the C-ABI translation
+// units.  Not installed; no symbol here is exported.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "doma_c.h"
+
+namespace doma
+{
+int fail(int code, const std::string & msg);
+int hip_fail(hipError_t e, const char * what);
+
+#define DOMA_HIP(call)                                                                           \
+  do                                                                                             \
+  {                                                                                              \
+    hipError_t _e = (call);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+      return ::doma::hip_fail(_e, #call);                                                        \
+  } while (0)
+
+// Arguments of the brick-wavefront sweep kernel (2D and 3D).
+struct BrickArgs
+{
+  int nx, ny, nz;          // cells per axis
+  int N;                   // nx*ny*nz
+  int dirs;                // directions per octant
+  int chunk;               // directions per direction set (<= DL)
+  int nchunks;             // direction sets per octant
+  int ndirsets;            // octants * nchunks
+  const double * dirtab;   // [noct*dirs][4] = |mu|, |eta|, |xi|, w
+  const double * lx;       // [nx]
+  const double * ly;       // [ny]
+  const double * lz;       // [nz]
+  const double * rlx;      // [nx] 1/lx (fast DD)
+  const double * rly;
+  const double * rlz;
+  const double * q;        // [N] iteration source
+  const double * sig;      // [N] sigma_t of the group being swept
+  double * part;           // [ndirsets][N] per-direction-set partial scalar flux
+  void * fx;               // global face buffers (scalar type of the sweep)
+  void * fy;
+  void * fz;
+  int nxp, nyp, nzp;       // extents padded to whole bricks
+  const int16_t * bricks;  // [nbricks][3] oriented brick coordinates, sorted by diagonal
+  int diag_begin;          // first brick of this launch's diagonal
+  int diag_count;          // bricks on the diagonal
+  const uint32_t * plane_cells;  // brick-local cells ordered by plane (li | lj<<8 | lk<<16)
+  const int32_t * plane_off;     // [nplanes+1]
+};
+
+struct Sweep1DArgs
+{
+  int nx;
+  int dirs;
+  const double * dirtab;  // [2*dirs][4]
+  const double * lx;
+  const double * q;
+  const double * sig;
+  double * part;          // [2*dirs][nx]
+};
+
+// Launch wrappers implemented in sweep.hip.
+int launch_brick_sweep(const BrickArgs & a, int dims, int scheme, int precision, int exact,
+                       int brick, int dl, int threads, hipStream_t s);
+int launch_sweep1d(const Sweep1DArgs & a, int precision, hipStream_t s);
+bool brick_variant_exists(int dims, int brick, int dl, int threads);
+
+// Launch wrappers implemented in iterate.hip.
+struct IterArgs
+{
+  int dims;
+  int N;
+  int nx, ny;
+  int G;
+  int g;
+  int ndirsets;
+  const int32_t * mat;
+  const double * sigt;     // [M][G]
+  const double * sigs;     // [M][G][G]
+  const double * src;      // [M][G]
+  const uint8_t * has_src; // [M]
+  const double * lx;
+  const double * ly;
+  const double * lz;
+  double * sig;            // [N] gathered sigma_t of group g
+  double * q;              // [N]
+  double * qmg;            // [N] (1D multigroup source)
+  double * phi;            // [G][N]
+  double * prev;           // [G][N]
+  double * part;           // [ndirsets][N]
+  double * dphi;           // [N] (used by the plain sweep entry point)
+  double * red;            // reduction partials
+  double * res;            // [4] device residual output
+};
+
+int launch_mg_source(const IterArgs & a, hipStream_t s);
+int launch_gather_sig(const IterArgs & a, hipStream_t s);
+int launch_finish(const IterArgs & a, int nblocks, hipStream_t s);
+int launch_residual_final(const IterArgs & a, int nblocks, hipStream_t s);
+int launch_reduce_parts(const IterArgs & a, hipStream_t s);
+int launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s);
+int launch_fill(double * p, size_t n, double v, hipStream_t s);
+int launch_set_q(const IterArgs & a, const double * qmg_dev, hipStream_t s);
+int reduction_blocks(int N);
+} // namespace doma
+
+struct doma_ctx
+{
+  int dims = 0, nx = 0, ny = 0, nz = 0, G = 0, scheme = 0, precision = 0, dirs = 0;
+  int noct = 0;
+  int N = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // geometry / quadrature (device)
+  double * d_lx = nullptr;
+  double * d_ly = nullptr;
+  double * d_lz = nullptr;
+  double * d_rlx = nullptr;
+  double * d_rly = nullptr;
+  double * d_rlz = nullptr;
+  double * d_dirtab = nullptr;
+
+  // materials (device)
+  int M = 0;
+  bool materials = false;
+  int32_t * d_mat = nullptr;
+  double * d_sigt = nullptr;
+  double * d_sigs = nullptr;
+  double * d_src = nullptr;
+  uint8_t * d_has_src = nullptr;
+
+  // iteration state (device)
+  double * d_sig = nullptr;
+  double * d_q = nullptr;
+  double * d_qmg = nullptr;
+  double * d_dphi = nullptr;
+  double * d_phi = nullptr;
+  double * d_prev = nullptr;
+  double * d_part = nullptr;
+  double * d_red = nullptr;
+  double * d_res = nullptr;
+  double * h_res = nullptr;  // pinned
+  int red_blocks = 0;
+
+  // sweep decomposition
+  doma_tuning tuning{};
+  int brick = 0, dl = 0, threads = 0, exact = 0;
+  int chunk = 0, nchunks = 0, ndirsets = 0;
+  int nxp = 0, nyp = 0, nzp = 0;
+  std::vector<int> diag_off;  // [ndiag+1]
+  int16_t * d_bricks = nullptr;
+  uint32_t * d_plane_cells = nullptr;
+  int32_t * d_plane_off = nullptr;
+  void * d_fx = nullptr;
+  void * d_fy = nullptr;
+  void * d_fz = nullptr;
+  size_t face_bytes = 0;
+  int launches_per_sweep = 0;
+  int autotuned = 0;
+
+  // Per-launch HIP events (doma_profile_sweeps); recorded only while profiling.
+  std::vector<hipEvent_t> prof_events;
+  bool profiling = false;
+};
+
+namespace doma
+{
+int configure_sweep(doma_ctx * c);
+int run_sweep(doma_ctx * c, int g);
+void free_sweep(doma_ctx * c);
+doma::IterArgs iter_args(doma_ctx * c, int g);
+} // namespace doma

@@ -1,0 +1,334 @@
+// iterate.hip -- source iteration and multigroup bookkeeping kernels.
+//
+// All reductions are deterministic: every block walks a fixed set of cells in a
+// fixed order and the block partials are summed by one block in index order, so
+// residuals (and therefore iteration counts) are reproducible run to run.
+// Arithmetic keeps the reference's expression order (no contraction) so that the
+// FP64 bookkeeping matches the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "internal.h"
+
+namespace doma
+{
+namespace
+{
+constexpr int RED_THREADS = 256;
+constexpr double PI = 3.14159265358979323846;
+
+// Angular normalisation of isotropic sources: 3D 0.25/pi (TransportSolver3D.cpp:
+// 351, 375, 542), 2D 0.5/pi (TransportSolver2D.cpp:351, 375, 542), 1D 0.5
+// (TransportSolver1D.cpp:384, 408, 814).
+__device__ __forceinline__ double
+norm_src(int dims, double a)
+{
+#pragma clang fp contract(off)
+  if (dims == 3)
+    return 0.25 * a / PI;
+  if (dims == 2)
+    return 0.5 * a / PI;
+  return 0.5 * a;
+}
+
+__device__ __forceinline__ double
+norm_term(int dims, double a, double b)
+{
+#pragma clang fp contract(off)
+  if (dims == 3)
+    return 0.25 * a * b / PI;
+  if (dims == 2)
+    return 0.5 * a * b / PI;
+  return 0.5 * a * b;
+}
+
+__device__ __forceinline__ double
+measure(const IterArgs & a, int c)
+{
+#pragma clang fp contract(off)
+  const int i = c % a.nx;
+  const int j = (c / a.nx) % a.ny;
+  const int k = c / (a.nx * a.ny);
+  if (a.dims == 3)
+    return a.lx[i] * a.ly[j] * a.lz[k];
+  if (a.dims == 2)
+    return a.lx[i] * a.ly[j];
+  return a.lx[i];
+}
+
+// Fixed-order block reduction of two sums; thread 0 gets the totals.
+__device__ __forceinline__ void
+block_sum2(double & u, double & v)
+{
+  __shared__ double su[RED_THREADS], sv[RED_THREADS];
+  su[threadIdx.x] = u;
+  sv[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1)
+  {
+    if ((int)threadIdx.x < s)
+    {
+      su[threadIdx.x] += su[threadIdx.x + s];
+      sv[threadIdx.x] += sv[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  u = su[0];
+  v = sv[0];
+}
+
+// updateMultigroupSource (TransportSolver3D.cpp:339-402; 2D :339-402; 1D
+// TransportSolver1D.cpp:369-435), fixed-source terms, fused with the gather of
+// the group's total cross section.
+__global__ void
+mg_source_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  const int m = a.mat[c];
+  const int G = a.G, g = a.g;
+  double acc = a.has_src[m] ? norm_src(a.dims, a.src[(size_t)m * G + g]) : 0.0;
+  for (int gp = 0; gp < G; ++gp)
+    if (gp != g)
+      acc += norm_term(a.dims, a.sigs[((size_t)m * G + g) * G + gp], a.phi[(size_t)gp * a.N + c]);
+  if (a.dims == 1)
+    a.qmg[c] = acc;
+  else
+    a.q[c] = acc;
+  a.prev[(size_t)g * a.N + c] = a.phi[(size_t)g * a.N + c];
+  a.phi[(size_t)g * a.N + c] = 0.0;
+  a.sig[c] = a.sigt[(size_t)m * G + g];
+}
+
+// Sum of the direction-set partials, the scattering residual terms and the
+// scattering-source update, fused.  2D/3D increment form: TransportSolver3D.cpp:
+// 531-563; 1D standard form: TransportSolver1D.cpp:802-838.
+__global__ __launch_bounds__(RED_THREADS) void
+finish_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  double diff = 0.0, total = 0.0;
+  const int stride = gridDim.x * RED_THREADS;
+  for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < a.N; c += stride)
+  {
+    double d = 0.0;
+    for (int s = 0; s < a.ndirsets; ++s)
+      d += a.part[(size_t)s * a.N + c];
+    const double v = measure(a, c);
+    const size_t pc = (size_t)a.g * a.N + c;
+    const double t = a.phi[pc];
+    const int m = a.mat[c];
+    const double sgg = a.sigs[((size_t)m * a.G + a.g) * a.G + a.g];
+    if (a.dims == 1)
+    {
+      diff = diff + (d - t) * (d - t) * v;
+      total = total + d * d * v;
+      a.phi[pc] = d;
+      double q = a.qmg[c];
+      q += 0.5 * sgg * d;
+      a.q[c] = q;
+    }
+    else
+    {
+      diff = diff + d * d * v;
+      total = total + (t + d) * (t + d) * v;
+      a.phi[pc] = t + d;
+      a.q[c] = norm_term(a.dims, sgg, d);
+    }
+  }
+  block_sum2(diff, total);
+  if (threadIdx.x == 0)
+  {
+    a.red[2 * blockIdx.x + 0] = diff;
+    a.red[2 * blockIdx.x + 1] = total;
+  }
+}
+
+// Final fixed-order sum of block partials; res[0] = residual as the reference
+// forms it (0 when the squared norm is <= 1e-8, TransportSolver3D.cpp:562).
+__global__ __launch_bounds__(RED_THREADS) void
+residual_final_kernel(IterArgs a, int nblocks)
+{
+  double diff = 0.0, total = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += RED_THREADS)
+  {
+    diff += a.red[2 * b + 0];
+    total += a.red[2 * b + 1];
+  }
+  block_sum2(diff, total);
+  if (threadIdx.x == 0)
+  {
+    a.res[0] = total > 1e-8 ? sqrt(diff) / sqrt(total) : 0.0;
+    a.res[1] = diff;
+    a.res[2] = total;
+  }
+}
+
+__global__ void
+reduce_parts_kernel(IterArgs a)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  double d = 0.0;
+  for (int s = 0; s < a.ndirsets; ++s)
+    d += a.part[(size_t)s * a.N + c];
+  a.dphi[c] = d;
+}
+
+// computeMGFluxResidual terms (TransportSolver3D.cpp:318-337).
+__global__ __launch_bounds__(RED_THREADS) void
+mg_residual_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  double diff = 0.0, total = 0.0;
+  const int stride = gridDim.x * RED_THREADS;
+  for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < a.N; c += stride)
+  {
+    const double v = measure(a, c);
+    for (int g = 0; g < a.G; ++g)
+    {
+      const double f = a.phi[(size_t)g * a.N + c];
+      const double d = f - a.prev[(size_t)g * a.N + c];
+      diff = diff + d * d * v;
+      total = total + f * f * v;
+    }
+  }
+  block_sum2(diff, total);
+  if (threadIdx.x == 0)
+  {
+    a.red[2 * blockIdx.x + 0] = diff;
+    a.red[2 * blockIdx.x + 1] = total;
+  }
+}
+
+__global__ void
+fill_kernel(double * p, size_t n, double v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+// Start of a single-group source iteration from a given multigroup source:
+// phi_g = 0 and q (2D/3D) or qmg (1D) = qmg_dev, with the group's cross sections.
+__global__ void
+set_q_kernel(IterArgs a, const double * qmg_dev)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  if (a.dims == 1)
+    a.qmg[c] = qmg_dev[c];
+  else
+    a.q[c] = qmg_dev[c];
+  a.phi[(size_t)a.g * a.N + c] = 0.0;
+  a.sig[c] = a.sigt[(size_t)a.mat[c] * a.G + a.g];
+}
+
+// 1D: sourceIteration applies updateScatteringSource once before the first
+// sweep (TransportSolver1D.cpp:500): phi_g = dphi (= 0), q = qmg.
+__global__ void
+q_from_qmg_kernel(IterArgs a)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  a.q[c] = a.qmg[c];
+  a.phi[(size_t)a.g * a.N + c] = 0.0;
+}
+
+// Gather of the swept group's total cross section for the plain sweep entry point.
+__global__ void
+gather_sig_kernel(IterArgs a)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  a.sig[c] = a.sigt[(size_t)a.mat[c] * a.G + a.g];
+}
+
+inline unsigned
+blocks_for(size_t n, int t)
+{
+  return (unsigned)((n + t - 1) / t);
+}
+} // namespace
+
+int
+reduction_blocks(int N)
+{
+  int b = (N + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+  if (b < 1)
+    b = 1;
+  if (b > 2048)
+    b = 2048;
+  return b;
+}
+
+int
+launch_mg_source(const IterArgs & a, hipStream_t s)
+{
+  hipLaunchKernelGGL(mg_source_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  if (a.dims == 1)
+    hipLaunchKernelGGL(q_from_qmg_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_set_q(const IterArgs & a, const double * qmg_dev, hipStream_t s)
+{
+  hipLaunchKernelGGL(set_q_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a, qmg_dev);
+  if (a.dims == 1)
+    hipLaunchKernelGGL(q_from_qmg_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_gather_sig(const IterArgs & a, hipStream_t s)
+{
+  hipLaunchKernelGGL(gather_sig_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_finish(const IterArgs & a, int nblocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(finish_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_residual_final(const IterArgs & a, int nblocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_reduce_parts(const IterArgs & a, hipStream_t s)
+{
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(mg_residual_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
+  hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_fill(double * p, size_t n, double v, hipStream_t s)
+{
+  if (n == 0)
+    return DOMA_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+  return DOMA_OK;
+}
+} // namespace doma

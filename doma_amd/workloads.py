@@ -1,0 +1,120 @@
+"""Named fixed-source workloads, built with the product's own quadrature and
+seeded-media generator (no oracle involved).
+
+* ``slab_five_region`` -- 1D, 8 cm slab, 400 uniform cells, five regions
+  (sigma_t 50/5/0/1/1, sigma_s 0/0/0/0.9/0.9, Q 50/0/0/1/0), S16 Gauss-Legendre,
+  diamond difference, one group.
+* ``random_blocks`` -- 3D cube of 1 cm cells with seeded 8^3-cell material
+  blocks: sigma_t ~ U[0.5, 2] per (block, group), c ~ U[0.3, 0.9], 0.7 of the
+  scattering in-group and 0.3 to the next group (the last group keeps only its
+  in-group share), unit source in group 0 on the central 8^3 cells; GC product
+  quadrature np x na per octant; theta-weighted DD.  64^3, 3 groups, 4x4 is the
+  standard instance.
+* ``multigroup_full`` -- 128^3, 8 groups, 8x4 per octant (256 directions),
+  log-uniform sigma_t in [0.2, 5], c ~ U[0.5, 0.95], full scattering matrix with
+  down-scatter and up-scatter between the last two groups, Q ~ U[0, 1] in 5% of
+  blocks.
+* ``high_scatter`` -- the same geometry with c ~ U[0.95, 0.995] and sigma_t
+  log-uniform in [0.5, 10].
+* ``batched_uneven`` -- independent seeded problems of uneven size for the
+  batched path.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+from .capi import MediaSpec, Problem
+
+SEED_BASE = 20261202
+
+
+def quadrature_3d(np_: int, na: int):
+    """GCAngularQuadrature(2na, 2np, 3) as the reference's 3D solver builds it."""
+    return capi.gc_quadrature(2 * na, 2 * np_, 3)
+
+
+def quadrature_2d(np_: int, na: int):
+    return capi.gc_quadrature(2 * na, 2 * np_, 2)
+
+
+def quadrature_1d(np_: int):
+    mu, w = capi.gl_quadrature(2 * np_)
+    return mu, np.zeros_like(mu), np.zeros_like(mu), w
+
+
+def slab_five_region(np_: int = 8, scale: int = 1) -> Problem:
+    """400*scale cells of 0.02/scale cm over the regions [0,2),[2,3),[3,5),[5,6),[6,8]."""
+    widths = [2.0, 1.0, 2.0, 1.0, 2.0]
+    cells = [100 * scale, 50 * scale, 100 * scale, 50 * scale, 100 * scale]
+    lx = np.concatenate([np.full(n, w / n) for w, n in zip(widths, cells)])
+    mat = np.concatenate([np.full(n, m, dtype=np.int32) for m, n in enumerate(cells)])
+    sigt = np.array([[50.0], [5.0], [0.0], [1.0], [1.0]])
+    sigs = np.array([0.0, 0.0, 0.0, 0.9, 0.9]).reshape(5, 1, 1)
+    src = np.array([[50.0], [0.0], [0.0], [1.0], [0.0]])
+    has = np.ones(5, dtype=np.uint8)
+    mu, eta, xi, w = quadrature_1d(np_)
+    return Problem(1, lx, np.ones(1), np.ones(1), 1, capi.SCHEME_DD, mat, sigt, sigs, src, has,
+                   mu, eta, xi, w, name=f"slab_five_region_{len(lx)}_S{2 * np_}")
+
+
+def _media(spec: MediaSpec):
+    return capi.media_generate(spec)
+
+
+def block_media_problem(n: int, groups: int, np_: int, na: int, scheme: int, spec: MediaSpec,
+                        name: str) -> Problem:
+    mat, sigt, sigs, src, has = _media(spec)
+    mu, eta, xi, w = quadrature_3d(np_, na)
+    l = np.full(n, spec.cell_width)
+    return Problem(3, l, l.copy(), l.copy(), groups, scheme, mat, sigt, sigs, src, has, mu, eta,
+                   xi, w, name=name, meta={"seed": int(spec.seed)})
+
+
+def random_blocks(n: int = 64, groups: int = 3, np_: int = 4, na: int = 4,
+                  scheme: int = capi.SCHEME_TW_DD, seed: int = SEED_BASE + 1,
+                  block: int = 8) -> Problem:
+    half = n // 2
+    spec = MediaSpec(n=n, block=block, groups=groups, sigt_log=0, sigt_lo=0.5, sigt_hi=2.0,
+                     c_lo=0.3, c_hi=0.9, scatter_rule=capi.SCATTER_DOWN1,
+                     source_rule=capi.SOURCE_CENTRAL, src_lo=half - 4, src_hi=half + 4,
+                     source_fraction=0.0, cell_width=1.0, seed=seed)
+    return block_media_problem(n, groups, np_, na, scheme, spec,
+                               f"random_blocks_{n}c_{groups}g_{8 * np_ * na}d")
+
+
+def multigroup_full(n: int = 128, groups: int = 8, np_: int = 8, na: int = 4,
+                    scheme: int = capi.SCHEME_TW_DD, seed: int = SEED_BASE + 2,
+                    c_lo: float = 0.5, c_hi: float = 0.95, sigt_lo: float = 0.2,
+                    sigt_hi: float = 5.0, block: int = 8) -> Problem:
+    spec = MediaSpec(n=n, block=block, groups=groups, sigt_log=1, sigt_lo=sigt_lo,
+                     sigt_hi=sigt_hi, c_lo=c_lo, c_hi=c_hi, scatter_rule=capi.SCATTER_FULL,
+                     source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0, source_fraction=0.05,
+                     cell_width=1.0, seed=seed)
+    return block_media_problem(n, groups, np_, na, scheme, spec,
+                               f"multigroup_full_{n}c_{groups}g_{8 * np_ * na}d")
+
+
+def high_scatter(n: int = 128, groups: int = 8, np_: int = 8, na: int = 4,
+                 seed: int = SEED_BASE + 4) -> Problem:
+    p = multigroup_full(n, groups, np_, na, capi.SCHEME_TW_DD, seed, c_lo=0.95, c_hi=0.995,
+                        sigt_lo=0.5, sigt_hi=10.0)
+    p.name = f"high_scatter_{n}c_{groups}g_{8 * np_ * na}d"
+    return p
+
+
+def batched_uneven(count: int = 512, seed: int = SEED_BASE + 3, np_: int = 4, na: int = 4,
+                   sizes=(16, 24, 32, 48, 64), groups=(1, 2, 4)):
+    """Independent problems: cube edge and group count drawn per problem."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        n = int(rng.choice(sizes))
+        g = int(rng.choice(groups))
+        spec = MediaSpec(n=n, block=8, groups=g, sigt_log=0, sigt_lo=0.5, sigt_hi=2.0, c_lo=0.9,
+                         c_hi=0.99, scatter_rule=capi.SCATTER_DOWN1,
+                         source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
+                         source_fraction=0.05, cell_width=1.0, seed=seed * 1000 + b)
+        out.append(block_media_problem(n, g, np_, na, capi.SCHEME_TW_DD, spec,
+                                       f"batched_{b}_{n}c_{g}g"))
+    return out

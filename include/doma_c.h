@@ -1,0 +1,200 @@
+/*
+ * doma_c.h -- C ABI of the MI355X-native discrete-ordinates sweep and
+ * scattering source iteration.
+ *
+ * Every entry point returns an int status (DOMA_OK on success) and never exits
+ * the process; doma_last_error() gives the message of the last failure on the
+ * calling thread.  No C++ or GPU-runtime types cross this boundary: meshes,
+ * materials and quadratures are plain arrays, host pointers are borrowed for
+ * the duration of a call, and device memory is owned by the context.  Calls on
+ * one context are serialised on the context's HIP stream; use one context per
+ * host thread.
+ *
+ * Reference seams replaced (paths relative to the nuclearkevin/doma checkout):
+ *   - T::solve policy + TransportSolver3D<T>::sweep (include/TransportSolver3D.h:
+ *     15-16, 76; src/TransportSolver3D.cpp:565-874; include/TWDiamondDifference3D.h:
+ *     15-92; include/DiamondDifference3D.h:14-49)      -> doma_sweep
+ *   - TransportSolver3D<T>::sourceIteration (src/TransportSolver3D.cpp:458-563)
+ *                                                       -> doma_source_iteration
+ *   - TransportSolver3D<T>::solveFixedSource (include/TransportSolver3D.h:28;
+ *     src/TransportSolver3D.cpp:262-402, 499-529)       -> doma_solve_fixed_source
+ *   - the 2D and 1D counterparts (src/TransportSolver2D.cpp:262-725,
+ *     src/TransportSolver1D.cpp:292-586, 802-915)       -> same entry points, dims 2/1
+ */
+#ifndef DOMA_C_H
+#define DOMA_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DOMA_OK 0
+#define DOMA_ERR_INVALID 1   /* bad argument or input deck semantics (reference: std::exit(1)) */
+#define DOMA_ERR_DEVICE 2    /* HIP runtime failure or missing device */
+#define DOMA_ERR_STATE 3     /* call order violated (e.g. solve before set_materials) */
+
+#define DOMA_SCHEME_DD 0     /* DiamondDifference{1,2,3}D */
+#define DOMA_SCHEME_TW_DD 1  /* TWDiamondDifference{2,3}D, theta = theta_s = 0.9 */
+
+#define DOMA_FP64 0          /* everything in FP64 */
+#define DOMA_FP32_SWEEP 1    /* FP32 angular flux + cell solve; FP64 scalar-flux
+                                accumulation, sources and residuals */
+
+#define DOMA_MAX_GROUPS 64
+
+typedef struct doma_ctx doma_ctx;
+
+/* Problem geometry, discretisation and angular quadrature.
+ * Cells are ordered k-major, i fastest: c = (k*ny + j)*nx + i, as
+ * BrickMesh3D (src/BrickMesh3D.cpp:86-102).  In 2D nz = 1, in 1D ny = nz = 1.
+ * Directions are [num_octants][dirs] in the reference's order: 8 octants
+ * PPP..MMM in 3D (GCAngularQuadrature(2na, 2np, 3)), 4 quadrants PPP, PMP,
+ * MPP, MMP in 2D (GCAngularQuadrature(2na, 2np, 2)), 2 half ranges (mu > 0,
+ * mu < 0) in 1D (GLAngularQuadrature(2np)).  Signed cosines; the sweeps use
+ * their magnitudes as the reference does. */
+typedef struct doma_problem_desc {
+  int32_t dims;
+  int32_t nx, ny, nz;
+  const double *lx, *ly, *lz;  /* [nx], [ny], [nz] cell widths (cm) */
+  int32_t groups;              /* 1 .. DOMA_MAX_GROUPS */
+  int32_t scheme;              /* DOMA_SCHEME_DD | DOMA_SCHEME_TW_DD (TW_DD: dims 2, 3) */
+  int32_t precision;           /* DOMA_FP64 | DOMA_FP32_SWEEP */
+  int32_t dirs;                /* directions per octant / quadrant / half range */
+  const double *mu, *eta, *xi, *w;
+} doma_problem_desc;
+
+/* Kernel shape.  Zero fields mean "library default" (or the autotuned choice
+ * when doma_autotune has been run on this context). */
+typedef struct doma_tuning {
+  int32_t brick;         /* 3D: brick edge in cells (4 or 8); 2D: tile edge (8 or 16) */
+  int32_t threads;       /* work-group size of the sweep kernel (64, 128, 256 or 512) */
+  int32_t exact_arith;   /* 1: reference expression order with IEEE divides,
+                            0: hoisted reciprocals (default) */
+  int32_t schedule;      /* 0: one launch per brick diagonal (default) */
+} doma_tuning;
+
+typedef struct doma_stats {
+  int32_t success;               /* reference solveFixedSource / sourceIteration result */
+  int32_t mg_iterations;
+  double mg_residual;
+  int32_t sweeps;                /* group sweeps performed */
+  int32_t si_iterations[DOMA_MAX_GROUPS];  /* SI count per group, last MG pass */
+  double si_residual[DOMA_MAX_GROUPS];
+  double elapsed_ms;             /* wall time of the call on the context stream */
+  double sweep_ms;               /* time spent in sweep kernels (HIP events) */
+} doma_stats;
+
+const char *doma_last_error(void);
+const char *doma_version(void);
+
+/* Create a context on the current HIP device and upload geometry/quadrature. */
+int doma_create(const doma_problem_desc *desc, doma_ctx **out);
+int doma_destroy(doma_ctx *ctx);
+
+/* Material map and tables (MaterialProps, include/InputParameters.h:24-55):
+ * mat[N] ids in [0, num_mats); sigt[M][G]; sigs[M][G][G] dest-major
+ * ([g_dest*G + g_src], TransportSolver3D.cpp:375); src[M][G]; has_src[M] = 0
+ * for a material without an external source (reference: empty _g_src). */
+int doma_set_materials(doma_ctx *ctx, const int32_t *mat, int32_t num_mats, const double *sigt,
+                       const double *sigs, const double *src, const uint8_t *has_src);
+
+int doma_set_tuning(doma_ctx *ctx, const doma_tuning *t);
+int doma_get_tuning(const doma_ctx *ctx, doma_tuning *t);
+
+/* One full sweep of group g over all octants: dphi[c] = sum_n w_n psi_n(c) for the
+ * per-cell iteration source q[c] (already scaled by the angular normalisation,
+ * e.g. 0.25/pi in 3D).  Host arrays of N doubles. */
+int doma_sweep(doma_ctx *ctx, int32_t g, const double *q, double *dphi);
+
+/* Same, on device arrays of N doubles, enqueued on the context stream. */
+int doma_sweep_device(doma_ctx *ctx, int32_t g, const double *d_q, double *d_dphi);
+
+/* Scattering source iteration of one group from phi_g = 0 with multigroup source
+ * qmg[N] (host).  Writes phi_g[N] (host, may be NULL), the iteration count and
+ * last residual; *converged is the reference's sourceIteration result. */
+int doma_source_iteration(doma_ctx *ctx, int32_t g, const double *qmg, double fs_tol,
+                          int32_t fs_max_it, double *phi_g, int32_t *iterations,
+                          double *residual, int32_t *converged);
+
+/* Full fixed-source solve: Gauss-Seidel over groups, each group solved by
+ * source iteration from zero (solveFixedSource semantics).  phi[G][N] (host) may
+ * be NULL; the flux stays resident and can be read with doma_get_flux. */
+int doma_solve_fixed_source(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, double mg_tol,
+                            int32_t mg_max_it, double *phi, doma_stats *stats);
+
+int doma_get_flux(doma_ctx *ctx, double *phi);
+
+/* Time n back-to-back sweeps of group g on the resident source (HIP events on
+ * the context stream); returns milliseconds per sweep and per-launch figures. */
+int doma_time_sweeps(doma_ctx *ctx, int32_t g, int32_t n, double *ms_per_sweep,
+                     int32_t *launches_per_sweep);
+
+/* Profile n sweeps of group g with a HIP event pair around every kernel launch on
+ * the context stream: kernel_ms = summed launch durations per sweep, wall_ms =
+ * sweep wall time on the stream, launches = kernel launches per sweep. */
+int doma_profile_sweeps(doma_ctx *ctx, int32_t g, int32_t n, double *kernel_ms, double *wall_ms,
+                        int32_t *launches);
+
+/* Number of cell-direction units one sweep of one group processes. */
+int64_t doma_units_per_sweep(const doma_ctx *ctx);
+
+/* Project-written autotuner: times the candidate kernel shapes on this
+ * context's problem and keeps the fastest (cached per problem shape). */
+int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
+
+/* ------------------------------------------------------------------------- */
+/* Angular quadratures (host only).                                            */
+/* ------------------------------------------------------------------------- */
+
+/* GCAngularQuadrature(n_c, n_l, dims) (src/GCAngularQuadrature.cpp:9-55): writes
+ * [8][n_c*n_l/4] (3D) or [4][n_c*n_l/4] (2D) signed directions and weights in the
+ * layout of doma_problem_desc.  The reference builds it with n_c = 2*na and
+ * n_l = 2*np (src/TransportSolver3D.cpp:10).  Returns directions per octant, or
+ * -1 (zero direction component for odd n_c / n_l, or a bad argument). */
+int doma_gc_quadrature(int32_t n_c, int32_t n_l, int32_t dims, double *mu, double *eta, double *xi,
+                       double *w);
+
+/* GLAngularQuadrature(n) (src/GLAngularQuadrature.cpp:3-20): n Legendre nodes,
+ * descending, so [0, n/2) is mu > 0.  Returns n/2, or -1. */
+int doma_gl_quadrature(int32_t n, double *mu, double *w);
+
+/* ------------------------------------------------------------------------- */
+/* Synthetic seeded media (counter-based SplitMix64, identical on every host).  */
+/* ------------------------------------------------------------------------- */
+
+#define DOMA_SCATTER_DOWN1 0  /* sigma_s[g][g] = 0.7 c sigma_t, [g+1][g] = 0.3 c sigma_t;
+                                 the last group keeps only its in-group part */
+#define DOMA_SCATTER_FULL 1   /* in-group plus down-scatter to every lower group,
+                                 up-scatter from the last group into the one above */
+#define DOMA_SOURCE_CENTRAL 0 /* unit source in group 0 on the cube [src_lo, src_hi)^3 */
+#define DOMA_SOURCE_BLOCKS 1  /* Q_g ~ U[0,1] in a seeded fraction of blocks */
+
+typedef struct doma_media_spec {
+  int32_t n;               /* cells per side of the cubic mesh */
+  int32_t block;           /* cells per side of a material block (divides n) */
+  int32_t groups;
+  int32_t sigt_log;        /* 0: sigma_t ~ U[lo, hi]; 1: log-uniform */
+  double sigt_lo, sigt_hi;
+  double c_lo, c_hi;       /* scattering ratio c ~ U[lo, hi] per (block, group) */
+  int32_t scatter_rule;
+  int32_t source_rule;
+  int32_t src_lo, src_hi;  /* DOMA_SOURCE_CENTRAL */
+  double source_fraction;  /* DOMA_SOURCE_BLOCKS */
+  double cell_width;       /* cm */
+  uint64_t seed;
+} doma_media_spec;
+
+/* Number of materials the spec produces (2 * blocks for DOMA_SOURCE_CENTRAL,
+ * blocks otherwise). */
+int32_t doma_media_num_materials(const doma_media_spec *spec);
+
+/* Fills mat[n^3], sigt[M][G], sigs[M][G][G], src[M][G], has_src[M]. Host only. */
+int doma_media_generate(const doma_media_spec *spec, int32_t *mat, double *sigt, double *sigs,
+                        double *src, uint8_t *has_src);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOMA_C_H */

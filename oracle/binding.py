@@ -1,0 +1,228 @@
+"""ctypes binding of the CPU oracle (oracle/libdoma_oracle.so).
+
+TEST INFRASTRUCTURE: only tests, the smoke check and the benchmark's CPU
+baseline use this module; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libdoma_oracle.so"
+REF_HARNESS = HERE / "_ref" / "ref_harness"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class OracleProblem(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("lx", _dp), ("ly", _dp), ("lz", _dp), ("groups", C.c_int), ("scheme", C.c_int),
+        ("num_mats", C.c_int), ("mat", _ip), ("sigt", _dp), ("sigs", _dp), ("src", _dp),
+        ("has_src", _u8p), ("dirs", C.c_int), ("mu", _dp), ("eta", _dp), ("xi", _dp),
+        ("w", _dp),
+    ]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [
+        ("success", C.c_int), ("mg_iterations", C.c_int), ("mg_residual", C.c_double),
+        ("sweeps", C.c_int), ("si_iterations_last", C.c_int * 64),
+        ("si_residual_last", C.c_double * 64),
+    ]
+
+
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        h.oracle_legendre.argtypes = [C.c_int, _dp, _dp]
+        h.oracle_chebyshev.argtypes = [C.c_int, _dp, _dp, _dp]
+        h.oracle_gc_quadrature.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        h.oracle_gl_quadrature.argtypes = [C.c_int, _dp, _dp]
+        for nm in ("oracle_twdd3d_cell", "oracle_dd3d_cell"):
+            f = getattr(h, nm)
+            f.restype = C.c_double
+            f.argtypes = [C.c_double] * 8 + [_dp, _dp]
+        h.oracle_twdd2d_cell.restype = C.c_double
+        h.oracle_twdd2d_cell.argtypes = [C.c_double] * 7 + [_dp, _dp]
+        h.oracle_dd2d_cell.restype = C.c_double
+        h.oracle_dd2d_cell.argtypes = [C.c_double] * 6 + [_dp, _dp]
+        h.oracle_dd1d_cell.restype = C.c_double
+        h.oracle_dd1d_cell.argtypes = [C.c_double] * 5 + [_dp]
+        h.oracle_sweep.argtypes = [C.POINTER(OracleProblem), C.c_int, _dp, _dp, C.c_int]
+        h.oracle_solve_fixed_source.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
+                                                C.c_double, C.c_int, C.c_int, _dp,
+                                                C.POINTER(OracleStats)]
+        h.oracle_source_iteration.argtypes = [C.POINTER(OracleProblem), C.c_int, _dp, C.c_double,
+                                              C.c_int, C.c_int, _dp, _ip, _dp, _dp]
+        _LIB = h
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class _Bound:
+    """Keeps the numpy arrays alive while the C struct points at them."""
+
+    def __init__(self, p):
+        p = p.normalized()
+        self.p = p
+        self.s = OracleProblem(p.dims, p.nx, p.ny, p.nz, _d(p.lx), _d(p.ly), _d(p.lz), p.groups,
+                               p.scheme, p.num_mats, p.mat.ctypes.data_as(_ip), _d(p.sigt),
+                               _d(p.sigs), _d(p.src), p.has_src.ctypes.data_as(_u8p), p.dirs,
+                               _d(p.mu), _d(p.eta), _d(p.xi), _d(p.w))
+
+
+def legendre(n):
+    r, w = np.zeros(n), np.zeros(n)
+    lib().oracle_legendre(n, _d(r), _d(w))
+    return r, w
+
+
+def gc_quadrature(n_c, n_l, dims):
+    noct = 8 if dims == 3 else 4
+    per = n_c * n_l // 4
+    a = [np.zeros(noct * max(per, 1)) for _ in range(4)]
+    r = lib().oracle_gc_quadrature(n_c, n_l, dims, *[_d(x) for x in a])
+    if r < 0:
+        raise ValueError("zero direction component")
+    return tuple(x.reshape(noct, per) for x in a)
+
+
+def gl_quadrature(n):
+    mu, w = np.zeros(n), np.zeros(n)
+    lib().oracle_gl_quadrature(n, _d(mu), _d(w))
+    return mu.reshape(2, n // 2), w.reshape(2, n // 2)
+
+
+def sweep(problem, g, q, nthreads=1):
+    b = _Bound(problem)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    out = np.zeros(b.p.N)
+    rc = lib().oracle_sweep(C.byref(b.s), g, _d(q), _d(out), nthreads)
+    if rc:
+        raise RuntimeError("oracle_sweep failed")
+    return out
+
+
+def solve_fixed_source(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1):
+    b = _Bound(problem)
+    phi = np.zeros(b.p.groups * b.p.N)
+    st = OracleStats()
+    ok = lib().oracle_solve_fixed_source(C.byref(b.s), fs_tol, fs_max_it, mg_tol, mg_max_it,
+                                         nthreads, _d(phi), C.byref(st))
+    if ok < 0:
+        raise RuntimeError("oracle_solve_fixed_source failed")
+    stats = {"success": bool(st.success), "mg_iterations": st.mg_iterations,
+             "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+             "si_iterations": list(st.si_iterations_last[: b.p.groups]),
+             "si_residual": list(st.si_residual_last[: b.p.groups])}
+    return phi.reshape(b.p.groups, b.p.N), stats
+
+
+def source_iteration(problem, g, qmg, fs_tol, fs_max_it, nthreads=1, trace=False):
+    b = _Bound(problem)
+    qmg = np.ascontiguousarray(qmg, dtype=np.float64).reshape(-1)
+    phi = np.zeros(b.p.N)
+    its, res = C.c_int32(), C.c_double()
+    tr = np.zeros(fs_max_it) if trace else None
+    conv = lib().oracle_source_iteration(C.byref(b.s), g, _d(qmg), fs_tol, fs_max_it, nthreads,
+                                         _d(phi), C.byref(its), C.byref(res),
+                                         _d(tr) if trace else None)
+    if conv < 0:
+        raise RuntimeError("oracle_source_iteration failed")
+    out = (phi, its.value, res.value, bool(conv))
+    return out + ((tr[: its.value],) if trace else ())
+
+
+def twdd3d_cell(q, sig, lx, ly, lz, amu, aeta, axi, inflow):
+    i = np.ascontiguousarray(inflow, dtype=np.float64)
+    o = np.zeros(3)
+    psi = lib().oracle_twdd3d_cell(q, sig, lx, ly, lz, amu, aeta, axi, _d(i), _d(o))
+    return psi, o
+
+
+def dd3d_cell(q, sig, lx, ly, lz, amu, aeta, axi, inflow):
+    i = np.ascontiguousarray(inflow, dtype=np.float64)
+    o = np.zeros(3)
+    psi = lib().oracle_dd3d_cell(q, sig, lx, ly, lz, amu, aeta, axi, _d(i), _d(o))
+    return psi, o
+
+
+def twdd2d_cell(q, sig, lx, ly, amu, aeta, axi, inflow):
+    i = np.ascontiguousarray(inflow, dtype=np.float64)
+    o = np.zeros(2)
+    psi = lib().oracle_twdd2d_cell(q, sig, lx, ly, amu, aeta, axi, _d(i), _d(o))
+    return psi, o
+
+
+def dd2d_cell(q, sig, lx, ly, amu, aeta, inflow):
+    i = np.ascontiguousarray(inflow, dtype=np.float64)
+    o = np.zeros(2)
+    psi = lib().oracle_dd2d_cell(q, sig, lx, ly, amu, aeta, _d(i), _d(o))
+    return psi, o
+
+
+def dd1d_cell(q, sig, lx, mu, inflow):
+    o = np.zeros(1)
+    psi = lib().oracle_dd1d_cell(q, sig, lx, mu, inflow, _d(o))
+    return psi, float(o[0])
+
+
+# --------------------------------------------------------------------------
+# Reference harness (oracle/_ref, built only where /root/reference exists).
+# --------------------------------------------------------------------------
+
+def write_problem_file(problem, path, np_, na, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1):
+    p = problem.normalized()
+    with open(path, "wb") as f:
+        f.write(struct.pack("<12i", p.dims, p.nx, p.ny, p.nz, p.groups, p.scheme, p.num_mats,
+                            np_, na, fs_max_it, mg_max_it, nthreads))
+        f.write(struct.pack("<2d", fs_tol, mg_tol))
+        for a in (p.lx, p.ly, p.lz):
+            f.write(a.astype("<f8").tobytes())
+        f.write(p.mat.astype("<i4").tobytes())
+        f.write(p.sigt.astype("<f8").tobytes())
+        f.write(p.sigs.astype("<f8").tobytes())
+        f.write(p.src.astype("<f8").tobytes())
+        f.write(p.has_src.astype("u1").tobytes())
+
+
+def run_reference(problem, np_, na, fs_tol, fs_max_it, mg_tol, mg_max_it, workdir, nthreads=1,
+                  tag="case"):
+    """Solve with the reference's own TransportSolver2D/3D; returns (ok, phi[G,N], log)."""
+    if not REF_HARNESS.exists():
+        raise RuntimeError("reference harness not built (make -C oracle ref)")
+    workdir = Path(workdir)
+    workdir.mkdir(parents=True, exist_ok=True)
+    pin = workdir / f"{tag}.bin"
+    pout = workdir / f"{tag}.out"
+    write_problem_file(problem, pin, np_, na, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads)
+    env = dict(os.environ, OMP_NUM_THREADS=str(nthreads))
+    subprocess.run([str(REF_HARNESS), str(pin), str(pout)], check=True, env=env)
+    raw = pout.read_bytes()
+    ok = struct.unpack("<i", raw[:4])[0]
+    p = problem.normalized()
+    phi = np.frombuffer(raw[4:], dtype="<f8").reshape(p.groups, p.N).copy()
+    log = Path(str(pout) + ".log").read_text()
+    return bool(ok), phi, log

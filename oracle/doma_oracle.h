@@ -1,0 +1,116 @@
+/*
+ * doma_oracle.h -- CPU restatement of DOMA's fixed-source discrete-ordinates
+ * sweep and scattering source iteration (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker for the HIP implementation and the timed CPU
+ * baseline.  Only tests, the smoke check and the benchmark's CPU-baseline leg
+ * may load it.  The product library never links or calls it.
+ *
+ * Every function follows the reference file:line quoted in its comment and
+ * keeps the reference's floating-point expression order, so that with the same
+ * OpenMP thread count it reproduces the reference bit for bit.
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   - quadrature: closed forms (numpy leggauss, Chebyshev nodes), weight sums,
+ *     antipodal symmetry, and the reference's own build (oracle/_ref);
+ *   - 3D/2D sweep + SI + MG: golden vectors written by the reference's own
+ *     compiled TransportSolver2D/3D units (tests/golden/), plus the per-cell
+ *     balance identity, DD linearity, TWDD(a=b=c=0.5) == DD, mirror symmetry;
+ *   - 1D sweep + SI + MG: the reference's 1D solver needs Eigen, which is absent,
+ *     so it cannot be built here; 1D is pinned by the exact DD recurrence in a
+ *     pure absorber, the cell balance identity, mirror symmetry of symmetric
+ *     decks and the infinite-medium limit.  The 1D DSA path is not restated.
+ */
+#ifndef DOMA_ORACLE_H
+#define DOMA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_DD = 0, ORACLE_TW_DD = 1 };
+
+typedef struct oracle_problem {
+  int dims;                 /* 1, 2 or 3 */
+  int nx, ny, nz;           /* cells per axis (ny = 1 in 1D, nz = 1 in 1D/2D) */
+  const double *lx;         /* [nx] cell widths along x */
+  const double *ly;         /* [ny] */
+  const double *lz;         /* [nz] */
+  int groups;
+  int scheme;               /* ORACLE_DD or ORACLE_TW_DD */
+  int num_mats;
+  const int32_t *mat;       /* [nx*ny*nz] material id per cell, k-major, i fastest */
+  const double *sigt;       /* [num_mats][groups] */
+  const double *sigs;       /* [num_mats][groups][groups], [dest*G + src] */
+  const double *src;        /* [num_mats][groups] */
+  const uint8_t *has_src;   /* [num_mats] 0: no source vector (reference: empty _g_src) */
+  int dirs;                 /* directions per octant (2D/3D) or per half range (1D) */
+  const double *mu;         /* [num_oct][dirs] signed direction cosines, reference order */
+  const double *eta;
+  const double *xi;
+  const double *w;          /* [num_oct][dirs] */
+} oracle_problem;
+
+typedef struct oracle_stats {
+  int success;              /* reference solveFixedSource return value */
+  int mg_iterations;
+  double mg_residual;
+  int sweeps;               /* total sweeps performed */
+  int si_iterations_last[64];  /* SI count per group in the last MG pass */
+  double si_residual_last[64];
+} oracle_stats;
+
+/* Quadrature (AngularPolynomials.cpp:8-111, GLAngularQuadrature.cpp:3-20,
+ * GCAngularQuadrature.cpp:9-100). */
+void oracle_legendre(int n, double *roots, double *weights);
+void oracle_chebyshev(int n, double *roots, double *angular_roots, double *weights);
+/* Fills mu/eta/xi/w as [8][n_c*n_l/4] (3D) or [4][n_c*n_l/4] (2D, octants with
+ * xi > 0 in PPP, PMP, MPP, MMP order).  Returns directions per octant, or -1. */
+int oracle_gc_quadrature(int n_c, int n_l, int dims, double *mu, double *eta, double *xi,
+                         double *w);
+/* 1D: mu/w as [2][n/2]: half 0 = mu > 0 (indices [0,n/2)), half 1 = mu < 0. */
+int oracle_gl_quadrature(int n, double *mu, double *w);
+
+/* Single-cell solves, for hand-checked values.
+ * TWDiamondDifference3D.h:15-92; DiamondDifference3D.h:14-49;
+ * TWDiamondDifference2D.h:15-76; DiamondDifference2D.h:14-45;
+ * DiamondDifference1D.h:14-46.  in[] are upwind faces, out[] downwind faces;
+ * returns the cell-centre angular flux. */
+double oracle_twdd3d_cell(double q, double sigt, double lx, double ly, double lz,
+                          double amu, double aeta, double axi, const double in[3],
+                          double out[3]);
+double oracle_dd3d_cell(double q, double sigt, double lx, double ly, double lz,
+                        double amu, double aeta, double axi, const double in[3],
+                        double out[3]);
+double oracle_twdd2d_cell(double q, double sigt, double lx, double ly, double amu,
+                          double aeta, double axi, const double in[2], double out[2]);
+double oracle_dd2d_cell(double q, double sigt, double lx, double ly, double amu,
+                        double aeta, const double in[2], double out[2]);
+double oracle_dd1d_cell(double q, double sigt, double lx, double mu, double in,
+                        double *out);
+
+/* One full sweep of group g (TransportSolver3D.cpp:565-653 and 655-874,
+ * TransportSolver2D.cpp:565-725, TransportSolver1D.cpp:840-915).
+ * q[N] is the per-cell iteration source; dphi[N] receives the swept scalar
+ * flux, reduced over per-thread partials in thread order. */
+int oracle_sweep(const oracle_problem *p, int g, const double *q, double *dphi, int nthreads);
+
+/* Full fixed-source solve (TransportSolver3D.cpp:262-563 and the 2D/1D
+ * counterparts).  phi_out is [G][N].  Returns the reference's boolean result. */
+int oracle_solve_fixed_source(const oracle_problem *p, double fs_tol, int fs_max_it,
+                              double mg_tol, int mg_max_it, int nthreads, double *phi_out,
+                              oracle_stats *stats);
+
+/* Source iteration of one group in isolation, starting from phi_tot[g] = 0 and
+ * the given multigroup source qmg[N] (already scaled by the angular
+ * normalisation).  Used to time sweeps and to trace residual sequences. */
+int oracle_source_iteration(const oracle_problem *p, int g, const double *qmg,
+                            double fs_tol, int fs_max_it, int nthreads, double *phi_g,
+                            int *iterations, double *residual, double *residual_trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

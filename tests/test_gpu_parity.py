@@ -1,0 +1,124 @@
+"""HIP path (through the C ABI) against the reference's golden vectors and the
+CPU oracle.  Bar: FP64 scalar flux within 1e-12 relative (max |d phi| / max |phi|
+per group) with identical SI/MG iteration counts; FP32-sweep mode within 1e-4
+of the FP64 result, with the measured error printed."""
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden, rel_err, solve_args
+from doma_amd import capi, workloads
+from doma_amd.capi import Context
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _solve(p, z, precision=capi.FP64, **tuning):
+    fs_tol, fs_max, mg_tol, mg_max = solve_args(z)
+    with Context(p, precision=precision, tuning=tuning or None) as ctx:
+        return ctx.solve_fixed_source(fs_tol, fs_max, mg_tol, mg_max)
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+@pytest.mark.parametrize("name", golden_names())
+def test_fixed_source_matches_reference(name, exact):
+    p, z, man = load_golden(name)
+    print(f"case {name} exact={exact} N={p.N} dirs={p.dirs * p.noct}", flush=True)
+    phi, st = _solve(p, z, exact_arith=exact)
+    assert st["success"] == bool(z["ref_success"])
+    ref = z["phi"] if "phi" in z.files else None
+    if ref is None:
+        idx = z["sample_idx"]
+        err = rel_err(phi[:, idx], z["phi_sample"])
+    else:
+        err = rel_err(phi, ref)
+    print(f"  max rel err per group {err}", flush=True)
+    assert np.all(err <= FP64_TOL)
+    if man["log"]["si"] and st["success"]:
+        last = {g: k for k, _, g in man["log"]["si"][-p.groups:]}
+        assert [last[g] for g in range(p.groups)] == st["si_iterations"]
+
+
+def _random_problem(dims, nx, ny, nz, groups, np_, na, scheme, seed):
+    rng = np.random.default_rng(seed)
+    M = 6
+    mat = rng.integers(0, M, size=nx * ny * nz).astype(np.int32)
+    sigt = rng.uniform(0.1, 4.0, (M, groups))
+    sigs = rng.uniform(0.0, 0.3, (M, groups, groups)) * sigt[:, None, :]
+    src = rng.uniform(0.0, 1.0, (M, groups))
+    has = (rng.uniform(size=M) < 0.7).astype(np.uint8)
+    has[0] = 1
+    if dims == 3:
+        mu, eta, xi, w = workloads.quadrature_3d(np_, na)
+    else:
+        mu, eta, xi, w = workloads.quadrature_2d(np_, na)
+    lz = rng.uniform(0.3, 2.0, nz) if dims == 3 else np.ones(1)
+    return capi.Problem(dims, rng.uniform(0.3, 2.0, nx), rng.uniform(0.3, 2.0, ny), lz, groups,
+                        scheme, mat, sigt, sigs, src, has, mu, eta, xi, w)
+
+
+SWEEP_SHAPES = [
+    # dims, nx, ny, nz, np, na, scheme, tuning
+    (3, 13, 9, 11, 2, 2, capi.SCHEME_TW_DD, dict(brick=4, threads=256)),
+    (3, 13, 9, 11, 2, 2, capi.SCHEME_DD, dict(brick=4, threads=128)),
+    (3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD, dict(brick=8, threads=256)),
+    (3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD, dict(brick=8, threads=512)),
+    (3, 10, 12, 9, 8, 4, capi.SCHEME_TW_DD, dict(brick=4, threads=256)),
+    (3, 8, 8, 8, 10, 10, capi.SCHEME_TW_DD, dict(brick=4, threads=256)),
+    (3, 8, 8, 8, 10, 10, capi.SCHEME_DD, dict(brick=8, threads=512)),
+    (2, 21, 18, 1, 3, 2, capi.SCHEME_TW_DD, dict(brick=8, threads=256)),
+    (2, 37, 29, 1, 4, 4, capi.SCHEME_DD, dict(brick=16, threads=256)),
+]
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+@pytest.mark.parametrize("shape", SWEEP_SHAPES, ids=lambda s: f"{s[0]}d_{s[1]}x{s[2]}x{s[3]}_"
+                         f"{s[4]}x{s[5]}_s{s[6]}_b{s[7]['brick']}t{s[7]['threads']}")
+def test_single_sweep_matches_oracle(shape, exact):
+    dims, nx, ny, nz, np_, na, scheme, tuning = shape
+    p = _random_problem(dims, nx, ny, nz, 2, np_, na, scheme, seed=nx * 1000 + ny)
+    rng = np.random.default_rng(5)
+    q = rng.uniform(0.0, 1.0, p.N) / (4 * np.pi)
+    q[rng.uniform(size=p.N) < 0.3] = 0.0  # exercise the 1e-8 TWDD branch
+    print(f"sweep shape {shape} exact={exact}", flush=True)
+    with Context(p, tuning=dict(tuning, exact_arith=exact)) as ctx:
+        for g in range(2):
+            got = ctx.sweep(g, q)
+            ref = ob.sweep(p, g, q, nthreads=1)
+            err = rel_err(got[None], ref[None])[0]
+            assert err <= 1e-13, err
+            if exact:
+                # Same IEEE operations per cell: only the direction-sum order differs.
+                assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-13
+
+
+def test_slab_five_region_1d():
+    p = workloads.slab_five_region()
+    fs = (1e-8, 10000, 1e-8, 10)
+    ref, rst = ob.solve_fixed_source(p, *fs)
+    with Context(p) as ctx:
+        phi, st = ctx.solve_fixed_source(*fs)
+    print("slab SI iterations", st["si_iterations"], "oracle", rst["si_iterations"], flush=True)
+    assert st["success"] and rst["success"]
+    assert st["si_iterations"] == rst["si_iterations"]
+    assert np.all(rel_err(phi, ref) <= FP64_TOL)
+
+
+def test_deterministic_bitwise():
+    p, z, _ = load_golden("random_blocks_16c_3g_128d")
+    a, _ = _solve(p, z)
+    b, _ = _solve(p, z)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["twdd3d_small", "random_blocks_16c_3g_128d", "twdd2d_small"])
+def test_fp32_sweep_error_statement(name):
+    p, z, _ = load_golden(name)
+    a, sa = _solve(p, z)
+    b, sb = _solve(p, z, precision=capi.FP32_SWEEP)
+    err = rel_err(b, a)
+    l2 = np.sqrt(((b - a) ** 2).sum(axis=1) / np.maximum((a ** 2).sum(axis=1), 1e-300))
+    print(f"FP32-sweep vs FP64 {name}: max rel per group {err}, L2 rel {l2}", flush=True)
+    assert np.all(err < 1e-4)
