@@ -105,6 +105,7 @@ iter_args(doma_ctx * c, int g)
   a.dphi = c->d_dphi;
   a.red = c->d_red;
   a.res = c->d_res;
+  a.err = c->d_err;
   return a;
 }
 
@@ -118,6 +119,32 @@ free_sweep(doma_ctx * c)
   dev_free(c->d_fy);
   dev_free(c->d_fz);
   dev_free(c->d_part);
+  dev_free(c->d_flags);
+}
+
+// Reports (and clears) an expired dependency wait of the dataflow schedule.
+int
+check_sweep_error(doma_ctx * c)
+{
+  int h = 0;
+  DOMA_HIP(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  if (h == 0)
+    return DOMA_OK;
+  return reset_after_sweep_error(c);
+}
+
+int
+reset_after_sweep_error(doma_ctx * c)
+{
+  DOMA_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  DOMA_HIP(hipMemsetAsync(c->d_ticket, 0, sizeof(unsigned long long), c->stream));
+  if (c->d_flags)
+    DOMA_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t) * (size_t)c->ntasks, c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  c->ticket_base = 0;
+  c->epoch = 0;
+  return fail(DOMA_ERR_DEVICE, "sweep dataflow wait expired: an upwind brick never finished");
 }
 
 // Brick schedule, hyperplane tables and face/partial buffers for the current
@@ -147,6 +174,7 @@ configure_sweep(doma_ctx * c)
   c->brick = B;
   c->threads = T;
   c->exact = c->tuning.exact_arith ? 1 : 0;
+  c->schedule = c->tuning.schedule == 1 ? 1 : 0;
   c->ndirsets = c->noct * c->nchunks;
 
   const int ntx = (c->nx + B - 1) / B, nty = (c->ny + B - 1) / B;
@@ -174,7 +202,11 @@ configure_sweep(doma_ctx * c)
       }
   }
   c->diag_off[ndiag] = (int)(bricks.size() / 3);
-  c->launches_per_sweep = ndiag;
+  c->ntx = ntx;
+  c->nty = nty;
+  c->ntz = ntz;
+  c->ntasks = (int)(bricks.size() / 3) * c->ndirsets;
+  c->launches_per_sweep = c->schedule == 1 ? 1 : ndiag;
 
   // Brick-local cells ordered by hyperplane li+lj+lk.
   const int bz = c->dims == 3 ? B : 1;
@@ -225,6 +257,22 @@ configure_sweep(doma_ctx * c)
   DOMA_HIP(hipMalloc(&c->d_fx, nfx * rs));
   DOMA_HIP(hipMalloc(&c->d_fy, nfy * rs));
   DOMA_HIP(hipMalloc(&c->d_fz, nfz * rs));
+
+  // Dataflow schedule: epoch flags per (direction set, brick), zeroed once; the
+  // persistent grid is as many work-groups as fit on the device at once.
+  if ((rc = dev_alloc(&c->d_flags, (size_t)c->ntasks)))
+    return rc;
+  DOMA_HIP(hipMemset(c->d_flags, 0, sizeof(uint32_t) * (size_t)c->ntasks));
+  c->epoch = 0;
+  if (c->schedule == 1)
+  {
+    int per_cu = 0;
+    if ((rc = flow_blocks_per_cu(c->dims, c->scheme, c->precision, c->exact, B, c->dl, T, &per_cu)))
+      return rc;
+    hipDeviceProp_t prop;
+    DOMA_HIP(hipGetDeviceProperties(&prop, c->device));
+    c->grid_flow = std::max(1, std::min(c->ntasks, std::max(per_cu, 1) * prop.multiProcessorCount));
+  }
   return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
 }
 
@@ -273,6 +321,31 @@ run_sweep_with(doma_ctx * c, const double * q)
   a.bricks = c->d_bricks;
   a.plane_cells = c->d_plane_cells;
   a.plane_off = c->d_plane_off;
+  a.ntx = c->ntx;
+  a.nty = c->nty;
+  a.ntz = c->ntz;
+  if (c->schedule == 1)
+  {
+    a.ntasks = c->ntasks;
+    a.flags = c->d_flags;
+    a.epoch = ++c->epoch;
+    a.ticket = c->d_ticket;
+    a.ticket_base = c->ticket_base;
+    a.error = c->d_err;
+    // 250 ms per dependency: orders of magnitude above any brick's latency.
+    a.wait_ticks = (long long)c->wall_clock_khz * 250;
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
+    int rc = launch_brick_sweep(a, c->dims, c->scheme, c->precision, c->exact, c->brick, c->dl,
+                                c->threads, 1, c->grid_flow, c->stream);
+    if (rc)
+      return rc;
+    // Every work-group takes one ticket past the last task before it exits.
+    c->ticket_base += (unsigned long long)c->ntasks + (unsigned long long)c->grid_flow;
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[1], c->stream));
+    return DOMA_OK;
+  }
   const int ndiag = (int)c->diag_off.size() - 1;
   for (int s = 0; s < ndiag; ++s)
   {
@@ -281,7 +354,7 @@ run_sweep_with(doma_ctx * c, const double * q)
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[2 * s], c->stream));
     int rc = launch_brick_sweep(a, c->dims, c->scheme, c->precision, c->exact, c->brick, c->dl,
-                                c->threads, c->stream);
+                                c->threads, 0, 0, c->stream);
     if (rc)
       return rc;
     if (c->profiling)
@@ -331,10 +404,12 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     if ((rc = launch_finish(a, c->red_blocks, c->stream)) ||
         (rc = launch_residual_final(a, c->red_blocks, c->stream)))
       return rc;
-    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 3 * sizeof(double), hipMemcpyDeviceToHost,
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 4 * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
     DOMA_HIP(hipStreamSynchronize(c->stream));
     DOMA_HIP(hipGetLastError());
+    if (c->h_res[3] != 0.0)
+      return reset_after_sweep_error(c);
     float ms = 0.f;
     DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (sweep_ms)
@@ -444,6 +519,12 @@ doma_create(const doma_problem_desc * d, doma_ctx ** out)
   };
   if (hipGetDevice(&c->device) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipGetDevice failed"));
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess &&
+        khz > 0)
+      c->wall_clock_khz = khz;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "stream/event creation failed"));
@@ -484,8 +565,13 @@ doma_create(const doma_problem_desc * d, doma_ctx ** out)
       (rc = dev_alloc(&c->d_prev, N * c->G)))
     return cleanup(rc);
   c->red_blocks = reduction_blocks(c->N);
-  if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 4)))
+  if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 4)) ||
+      (rc = dev_alloc(&c->d_err, 1)) || (rc = dev_alloc(&c->d_ticket, 1)))
     return cleanup(rc);
+  if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(c->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->d_res, 0, 4 * sizeof(double)) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "hipMemset failed"));
   if (hipHostMalloc(reinterpret_cast<void **>(&c->h_res), 4 * sizeof(double)) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
   if (hipMemcpy(c->d_lx, lxv.data(), lxv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -546,6 +632,8 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_prev);
   dev_free(c->d_red);
   dev_free(c->d_res);
+  dev_free(c->d_err);
+  dev_free(c->d_ticket);
   if (c->h_res)
     (void)hipHostFree(c->h_res);
   if (c->ev0)
@@ -635,7 +723,7 @@ doma_get_tuning(const doma_ctx * c, doma_tuning * t)
   t->brick = c->brick;
   t->threads = c->threads;
   t->exact_arith = c->exact;
-  t->schedule = 0;
+  t->schedule = c->schedule;
   return DOMA_OK;
 }
 
@@ -670,7 +758,7 @@ doma_sweep(doma_ctx * c, int32_t g, const double * q, double * dphi)
     return rc;
   DOMA_HIP(hipMemcpyAsync(dphi, c->d_dphi, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
   DOMA_HIP(hipStreamSynchronize(c->stream));
-  return DOMA_OK;
+  return check_sweep_error(c);
 }
 
 extern "C" int
@@ -811,6 +899,8 @@ doma_time_sweeps(doma_ctx * c, int32_t g, int32_t n, double * ms_per_sweep, int3
   DOMA_HIP(hipEventRecord(c->ev1, c->stream));
   DOMA_HIP(hipEventSynchronize(c->ev1));
   DOMA_HIP(hipGetLastError());
+  if ((rc = check_sweep_error(c)))
+    return rc;
   float ms = 0.f;
   DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   if (ms_per_sweep)
@@ -865,6 +955,8 @@ doma_profile_sweeps(doma_ctx * c, int32_t g, int32_t n, double * kernel_ms, doub
       ksum += ms;
   }
   DOMA_HIP(hipGetLastError());
+  if ((rc = check_sweep_error(c)))
+    return rc;
   if (kernel_ms)
     *kernel_ms = ksum / n;
   if (wall_ms)
@@ -886,10 +978,12 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
   if (c->dims == 3)
     for (int b : {4, 8})
       for (int t : {128, 256, 512})
-        cands.push_back(doma_tuning{b, t, 0, 0});
+        for (int sch : {0, 1})
+          cands.push_back(doma_tuning{b, t, 0, sch});
   else if (c->dims == 2)
     for (int b : {8, 16})
-      cands.push_back(doma_tuning{b, 256, 0, 0});
+      for (int sch : {0, 1})
+        cands.push_back(doma_tuning{b, 256, 0, sch});
   else
     cands.push_back(doma_tuning{0, 0, 0, 0});
   const doma_tuning keep = c->tuning;

@@ -1,0 +1,199 @@
+// cellsolve.h -- per-cell discrete-ordinates solves (device functions).
+//
+// Cell solves: TWDiamondDifference3D.h:15-92, DiamondDifference3D.h:14-49,
+// TWDiamondDifference2D.h:15-76, DiamondDifference2D.h:14-45,
+// DiamondDifference1D.h:14-46.  The "exact" variants keep the reference's
+// expression order with IEEE divides and no contraction; the "fast" variants
+// hoist the weight reciprocals (1/a = d/(d-n) with the same clamp) and use FMA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "doma_c.h"
+
+namespace doma
+{
+// ------------------------------------------------------------------------- //
+// Cell solves                                                                //
+// ------------------------------------------------------------------------- //
+
+template <typename R>
+__device__ __forceinline__ R
+twdd3d_exact(R q, R sig, R lx, R ly, R lz, R amu, R aeta, R axi, R & x, R & y, R & z)
+{
+#pragma clang fp contract(off)
+  const R theta = R(0.9), theta_s = R(0.9);
+  const R A = ly * lz;
+  const R B = lx * lz;
+  const R C = lx * ly;
+  const R V = lx * ly * lz;
+  R a = R(0.5);
+  if (x > R(1e-8))
+  {
+    a = q * V * theta_s + (aeta * B * y + axi * C * z) * theta + amu * A * x;
+    a /= (sig * V + R(2.0) * aeta * B + R(2.0) * axi * C) * x;
+    a = R(-1.0) * (a - R(1.0));
+  }
+  R b = R(0.5);
+  if (y > R(1e-8))
+  {
+    b = q * V * theta_s + (amu * A * x + axi * C * z) * theta + aeta * B * y;
+    b /= (sig * V + R(2.0) * amu * A + R(2.0) * axi * C) * y;
+    b = R(-1.0) * (b - R(1.0));
+  }
+  R c = R(0.5);
+  if (z > R(1e-8))
+  {
+    c = q * V * theta_s + (amu * A * x + aeta * B * y) * theta + axi * C * z;
+    c /= (sig * V + R(2.0) * amu * A + R(2.0) * aeta * B) * z;
+    c = R(-1.0) * (c - R(1.0));
+  }
+  a = a < R(0.5) ? R(0.5) : a;
+  a = a > R(1.0) ? R(1.0) : a;
+  b = b < R(0.5) ? R(0.5) : b;
+  b = b > R(1.0) ? R(1.0) : b;
+  c = c < R(0.5) ? R(0.5) : c;
+  c = c > R(1.0) ? R(1.0) : c;
+  R psi = q * V + (amu * A / a * x) + (aeta * B / b * y) + (axi * C / c * z);
+  psi /= sig * V + (amu * A / a) + (aeta * B / b) + (axi * C / c);
+  x = (psi / a) - (((R(1.0) - a) / a) * x);
+  y = (psi / b) - (((R(1.0) - b) / b) * y);
+  z = (psi / c) - (((R(1.0) - c) / c) * z);
+  return psi;
+}
+
+// Inverse TWDD weight 1/a for a = 1 - n/d clamped to [0.5, 1]: a < 0.5 <=> n > d/2
+// and a > 1 <=> n < 0 (d > 0 whenever the upwind flux exceeds 1e-8).
+template <typename R>
+__device__ __forceinline__ R
+inv_weight(R n, R d)
+{
+  R r = d / (d - n);
+  r = n < R(0) ? R(1) : r;
+  r = n >= R(0.5) * d ? R(2) : r;
+  return r;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+twdd3d_fast(R qV, R sV, R mA, R eB, R xC, R & x, R & y, R & z)
+{
+  const R th = R(0.9);
+  const R qVs = qV * th;
+  const R mAx = mA * x, eBy = eB * y, xCz = xC * z;
+  R ia = R(2), ib = R(2), ic = R(2);
+  if (x > R(1e-8))
+    ia = inv_weight(qVs + (eBy + xCz) * th + mAx, (sV + R(2) * (eB + xC)) * x);
+  if (y > R(1e-8))
+    ib = inv_weight(qVs + (mAx + xCz) * th + eBy, (sV + R(2) * (mA + xC)) * y);
+  if (z > R(1e-8))
+    ic = inv_weight(qVs + (mAx + eBy) * th + xCz, (sV + R(2) * (mA + eB)) * z);
+  const R wa = mA * ia, wb = eB * ib, wc = xC * ic;
+  const R psi = (qV + wa * x + wb * y + wc * z) / (sV + wa + wb + wc);
+  x = ia * psi - (ia - R(1)) * x;
+  y = ib * psi - (ib - R(1)) * y;
+  z = ic * psi - (ic - R(1)) * z;
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+dd3d_exact(R q, R sig, R lx, R ly, R lz, R amu, R aeta, R axi, R & x, R & y, R & z)
+{
+#pragma clang fp contract(off)
+  R psi = q + (R(2.0) * x * amu / lx) + (R(2.0) * y * aeta / ly) + (R(2.0) * z * axi / lz);
+  psi /= (R(2.0) * amu / lx) + (R(2.0) * aeta / ly) + (R(2.0) * axi / lz) + sig;
+  x = R(2.0) * psi - x;
+  y = R(2.0) * psi - y;
+  z = R(2.0) * psi - z;
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+dd3d_fast(R q, R sig, R cx, R cy, R cz, R & x, R & y, R & z)
+{
+  const R psi = (q + cx * x + cy * y + cz * z) / (cx + cy + cz + sig);
+  x = R(2) * psi - x;
+  y = R(2) * psi - y;
+  z = R(2) * psi - z;
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+twdd2d_exact(R q, R sig, R lx, R ly, R amu, R aeta, R axi, R & x, R & y)
+{
+#pragma clang fp contract(off)
+  const R theta = R(0.9), theta_s = R(0.9);
+  const R A = ly;
+  const R B = lx;
+  const R V = lx * ly;
+  R a = R(0.5);
+  if (x > R(1e-8))
+  {
+    a = q * V * theta_s + (aeta * B * y) * theta + amu * A * x;
+    a /= (sig * V + R(2.0) * aeta * B) * x;
+    a = R(-1.0) * (a - R(1.0));
+  }
+  R b = R(0.5);
+  if (y > R(1e-8))
+  {
+    b = q * V * theta_s + (amu * A * x + axi) * theta + aeta * B * y;
+    b /= (sig * V + R(2.0) * amu * A) * y;
+    b = R(-1.0) * (b - R(1.0));
+  }
+  a = a < R(0.5) ? R(0.5) : a;
+  a = a > R(1.0) ? R(1.0) : a;
+  b = b < R(0.5) ? R(0.5) : b;
+  b = b > R(1.0) ? R(1.0) : b;
+  R psi = q * V + (amu * A / a * x) + (aeta * B / b * y);
+  psi /= sig * V + (amu * A / a) + (aeta * B / b);
+  x = (psi / a) - (((R(1.0) - a) / a) * x);
+  y = (psi / b) - (((R(1.0) - b) / b) * y);
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+twdd2d_fast(R qV, R sV, R mA, R eB, R axi, R & x, R & y)
+{
+  const R th = R(0.9);
+  const R qVs = qV * th;
+  const R mAx = mA * x, eBy = eB * y;
+  R ia = R(2), ib = R(2);
+  if (x > R(1e-8))
+    ia = inv_weight(qVs + eBy * th + mAx, (sV + R(2) * eB) * x);
+  if (y > R(1e-8))
+    ib = inv_weight(qVs + (mAx + axi) * th + eBy, (sV + R(2) * mA) * y);
+  const R wa = mA * ia, wb = eB * ib;
+  const R psi = (qV + wa * x + wb * y) / (sV + wa + wb);
+  x = ia * psi - (ia - R(1)) * x;
+  y = ib * psi - (ib - R(1)) * y;
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+dd2d_exact(R q, R sig, R lx, R ly, R amu, R aeta, R & x, R & y)
+{
+#pragma clang fp contract(off)
+  R psi = q + (R(2.0) * x * amu / lx) + (R(2.0) * y * aeta / ly);
+  psi /= (R(2.0) * amu / lx) + (R(2.0) * aeta / ly) + sig;
+  x = R(2.0) * psi - x;
+  y = R(2.0) * psi - y;
+  return psi;
+}
+
+template <typename R>
+__device__ __forceinline__ R
+dd1d(R q, R sig, R lx, R amu, R & x)
+{
+#pragma clang fp contract(off)
+  R psi = q + (R(2.0) * x * amu / lx);
+  psi /= (R(2.0) * amu / lx) + sig;
+  x = R(2.0) * psi - x;
+  return psi;
+}
+
+} // namespace doma

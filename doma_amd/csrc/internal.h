@@ -46,11 +46,21 @@ struct BrickArgs
   void * fy;
   void * fz;
   int nxp, nyp, nzp;       // extents padded to whole bricks
+  int ntx, nty, ntz;       // bricks per axis
   const int16_t * bricks;  // [nbricks][3] oriented brick coordinates, sorted by diagonal
   int diag_begin;          // first brick of this launch's diagonal
   int diag_count;          // bricks on the diagonal
   const uint32_t * plane_cells;  // brick-local cells ordered by plane (li | lj<<8 | lk<<16)
   const int32_t * plane_off;     // [nplanes+1]
+
+  // Dataflow schedule (one persistent launch per sweep).
+  int ntasks;                       // nbricks * ndirsets, ticket order = diagonal order
+  uint32_t * flags;                 // [ndirsets][nbricks] epoch of the last finished brick
+  uint32_t epoch;                   // this sweep's epoch (> every flag value before it)
+  unsigned long long * ticket;      // monotone ticket counter (never reset)
+  unsigned long long ticket_base;   // counter value at the start of this sweep
+  int * error;                      // set when a bounded wait expires
+  long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
 };
 
 struct Sweep1DArgs
@@ -66,7 +76,9 @@ struct Sweep1DArgs
 
 // Launch wrappers implemented in sweep.hip.
 int launch_brick_sweep(const BrickArgs & a, int dims, int scheme, int precision, int exact,
-                       int brick, int dl, int threads, hipStream_t s);
+                       int brick, int dl, int threads, int schedule, int grid_flow, hipStream_t s);
+int flow_blocks_per_cu(int dims, int scheme, int precision, int exact, int brick, int dl,
+                       int threads, int * per_cu);
 int launch_sweep1d(const Sweep1DArgs & a, int precision, hipStream_t s);
 bool brick_variant_exists(int dims, int brick, int dl, int threads);
 
@@ -95,7 +107,8 @@ struct IterArgs
   double * part;           // [ndirsets][N]
   double * dphi;           // [N] (used by the plain sweep entry point)
   double * red;            // reduction partials
-  double * res;            // [4] device residual output
+  double * res;            // [4] residual, diff, total, sweep error flag
+  const int * err;         // sweep error flag (bounded wait expired)
 };
 
 int launch_mg_source(const IterArgs & a, hipStream_t s);
@@ -168,12 +181,26 @@ struct doma_ctx
   // Per-launch HIP events (doma_profile_sweeps); recorded only while profiling.
   std::vector<hipEvent_t> prof_events;
   bool profiling = false;
+
+  // Dataflow schedule state.
+  int schedule = 0;
+  int ntx = 0, nty = 0, ntz = 0;
+  int ntasks = 0;
+  int grid_flow = 0;
+  uint32_t * d_flags = nullptr;
+  unsigned long long * d_ticket = nullptr;
+  int * d_err = nullptr;
+  uint32_t epoch = 0;
+  unsigned long long ticket_base = 0;
+  int wall_clock_khz = 100000;
 };
 
 namespace doma
 {
 int configure_sweep(doma_ctx * c);
 int run_sweep(doma_ctx * c, int g);
+int check_sweep_error(doma_ctx * c);
+int reset_after_sweep_error(doma_ctx * c);
 void free_sweep(doma_ctx * c);
 doma::IterArgs iter_args(doma_ctx * c, int g);
 } // namespace doma

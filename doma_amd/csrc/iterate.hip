@@ -164,6 +164,7 @@ residual_final_kernel(IterArgs a, int nblocks)
     a.res[0] = total > 1e-8 ? sqrt(diff) / sqrt(total) : 0.0;
     a.res[1] = diff;
     a.res[2] = total;
+    a.res[3] = a.err ? static_cast<double>(*a.err) : 0.0;
   }
 }
 

@@ -73,17 +73,18 @@ SWEEP_SHAPES = [
 ]
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("exact", [0, 1])
 @pytest.mark.parametrize("shape", SWEEP_SHAPES, ids=lambda s: f"{s[0]}d_{s[1]}x{s[2]}x{s[3]}_"
                          f"{s[4]}x{s[5]}_s{s[6]}_b{s[7]['brick']}t{s[7]['threads']}")
-def test_single_sweep_matches_oracle(shape, exact):
+def test_single_sweep_matches_oracle(shape, exact, schedule):
     dims, nx, ny, nz, np_, na, scheme, tuning = shape
     p = _random_problem(dims, nx, ny, nz, 2, np_, na, scheme, seed=nx * 1000 + ny)
     rng = np.random.default_rng(5)
     q = rng.uniform(0.0, 1.0, p.N) / (4 * np.pi)
     q[rng.uniform(size=p.N) < 0.3] = 0.0  # exercise the 1e-8 TWDD branch
-    print(f"sweep shape {shape} exact={exact}", flush=True)
-    with Context(p, tuning=dict(tuning, exact_arith=exact)) as ctx:
+    print(f"sweep shape {shape} exact={exact} schedule={schedule}", flush=True)
+    with Context(p, tuning=dict(tuning, exact_arith=exact, schedule=schedule)) as ctx:
         for g in range(2):
             got = ctx.sweep(g, q)
             ref = ob.sweep(p, g, q, nthreads=1)
@@ -106,11 +107,29 @@ def test_slab_five_region_1d():
     assert np.all(rel_err(phi, ref) <= FP64_TOL)
 
 
-def test_deterministic_bitwise():
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_deterministic_bitwise(schedule):
+    """Fixed reduction order: repeated solves are bitwise identical, and the
+    dataflow and diagonal schedules agree bitwise (same per-cell arithmetic)."""
     p, z, _ = load_golden("random_blocks_16c_3g_128d")
-    a, _ = _solve(p, z)
-    b, _ = _solve(p, z)
+    a, _ = _solve(p, z, schedule=schedule)
+    b, _ = _solve(p, z, schedule=schedule)
     np.testing.assert_array_equal(a, b)
+    c, _ = _solve(p, z, schedule=1 - schedule)
+    np.testing.assert_array_equal(a, c)
+
+
+@pytest.mark.parametrize("name", ["random_blocks_64c_3g_128d", "kobayashi1_20c_800d_twdd",
+                                  "twdd2d_small", "random_blocks_24c_3g_32d_dd"])
+def test_dataflow_schedule_matches_reference(name):
+    p, z, man = load_golden(name)
+    phi, st = _solve(p, z, schedule=1)
+    ref = z["phi"] if "phi" in z.files else None
+    err = rel_err(phi, ref) if ref is not None else rel_err(phi[:, z["sample_idx"]],
+                                                            z["phi_sample"])
+    print(f"dataflow {name}: max rel err per group {err}", flush=True)
+    assert st["success"] == bool(z["ref_success"])
+    assert np.all(err <= FP64_TOL)
 
 
 @pytest.mark.parametrize("name", ["twdd3d_small", "random_blocks_16c_3g_128d", "twdd2d_small"])
