@@ -47,7 +47,11 @@ class ProblemDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("brick", C.c_int32), ("threads", C.c_int32), ("exact_arith", C.c_int32),
-                ("schedule", C.c_int32)]
+                ("schedule", C.c_int32), ("octants", C.c_int32), ("dirs_per_lane", C.c_int32),
+                ("shape", C.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -249,15 +253,16 @@ class Context:
         except Exception:
             pass
 
-    def set_tuning(self, brick=0, threads=0, exact_arith=0, schedule=0):
-        t = Tuning(brick, threads, exact_arith, schedule)
+    def set_tuning(self, brick=0, threads=0, exact_arith=0, schedule=0, octants=0,
+                   dirs_per_lane=0, shape=0):
+        """Restrict the sweep kernel shape (0 = any); shape is the kernel id + 1."""
+        t = Tuning(brick, threads, exact_arith, schedule, octants, dirs_per_lane, shape)
         _check(lib().doma_set_tuning(self.h, C.byref(t)), "doma_set_tuning")
 
     def get_tuning(self) -> dict:
         t = Tuning()
         _check(lib().doma_get_tuning(self.h, C.byref(t)), "doma_get_tuning")
-        return {"brick": t.brick, "threads": t.threads, "exact_arith": t.exact_arith,
-                "schedule": t.schedule}
+        return t.as_dict()
 
     def sweep(self, g: int, q: np.ndarray) -> np.ndarray:
         q = _f64(q).reshape(-1)
@@ -316,7 +321,7 @@ class Context:
     def autotune(self, g: int = 0):
         t, ms = Tuning(), C.c_double()
         _check(lib().doma_autotune(self.h, g, C.byref(t), C.byref(ms)), "doma_autotune")
-        return {"brick": t.brick, "threads": t.threads, "exact_arith": t.exact_arith}, ms.value
+        return t.as_dict(), ms.value
 
 
 def media_generate(spec: MediaSpec):

@@ -147,6 +147,50 @@ reset_after_sweep_error(doma_ctx * c)
   return fail(DOMA_ERR_DEVICE, "sweep dataflow wait expired: an upwind brick never finished");
 }
 
+// Kernel shape for a problem: shapes of the right dimension whose direction
+// width covers the chunk, whose octant group divides the octant count and that
+// agree with every non-zero tuning hint.  Without hints the default prefers the
+// narrowest direction width, then 8^3 bricks on large grids (4^3 otherwise; 16^2
+// / 8^2 tiles in 2D), then one direction per lane and one octant per group
+// (measured fastest on MI355X: the sweep is latency bound, and those shapes
+// give the shortest per-brick critical path at the highest occupancy).
+int
+select_shape(int dims, int noct, int chunk, int minext, const doma_tuning & t)
+{
+  if (t.shape > 0)
+  {
+    ShapeInfo si{};
+    if (!shape_info(t.shape - 1, &si) || si.dims != dims || si.dl * si.ndl < chunk ||
+        noct % si.ko != 0)
+      return -1;
+    return t.shape - 1;
+  }
+  const int pref_brick = dims == 3 ? (minext >= 96 ? 8 : 4) : (minext >= 128 ? 16 : 8);
+  int best = -1;
+  long best_score = 0;
+  for (int s = 0; s < num_shapes(); ++s)
+  {
+    ShapeInfo si{};
+    shape_info(s, &si);
+    const int dw = si.dl * si.ndl;
+    if (si.dims != dims || dw < chunk || noct % si.ko != 0)
+      continue;
+    if ((t.brick && si.brick != t.brick) || (t.threads && si.threads != t.threads) ||
+        (t.octants && si.ko != t.octants) || (t.dirs_per_lane && si.ndl != t.dirs_per_lane))
+      continue;
+    const int pref_threads = si.brick == 8 ? 512 : 256;
+    const long score = (long)(dw - chunk) * 1000 + (si.brick != pref_brick ? 100 : 0) +
+                       (si.ndl == 1 ? 0 : 20) + (si.ko - 1) * 4 +
+                       (si.threads != pref_threads ? 1 : 0);
+    if (best < 0 || score < best_score)
+    {
+      best = s;
+      best_score = score;
+    }
+  }
+  return best;
+}
+
 // Brick schedule, hyperplane tables and face/partial buffers for the current
 // tuning.  1D uses one thread per direction and no bricks.
 int
@@ -160,19 +204,21 @@ configure_sweep(doma_ctx * c)
     return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
   }
   const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
-  int B = c->tuning.brick;
-  if (B == 0)
-    B = c->dims == 3 ? (minext >= 48 ? 8 : 4) : (minext >= 64 ? 16 : 8);
-  int T = c->tuning.threads ? c->tuning.threads : 256;
   c->nchunks = (c->dirs + 63) / 64;
   c->chunk = (c->dirs + c->nchunks - 1) / c->nchunks;
-  c->dl = std::max(8, next_pow2(c->chunk));
-  if (!brick_variant_exists(c->dims, B, c->dl, T))
-    return fail(DOMA_ERR_INVALID, "no sweep kernel for brick " + std::to_string(B) + ", " +
-                                      std::to_string(T) + " threads, " + std::to_string(c->dl) +
-                                      " direction lanes");
+  const int shape = select_shape(c->dims, c->noct, c->chunk, minext, c->tuning);
+  ShapeInfo si{};
+  if (shape < 0 || !shape_info(shape, &si))
+    return fail(DOMA_ERR_INVALID, "no sweep kernel shape matches the tuning for " +
+                                      std::to_string(c->chunk) + " directions per chunk");
+  const int B = si.brick;
+  c->shape = shape;
   c->brick = B;
-  c->threads = T;
+  c->threads = si.threads;
+  c->dl = si.dl;
+  c->ndl = si.ndl;
+  c->ko = si.ko;
+  c->dw = si.dl * si.ndl;
   c->exact = c->tuning.exact_arith ? 1 : 0;
   c->schedule = c->tuning.schedule == 1 ? 1 : 0;
   c->ndirsets = c->noct * c->nchunks;
@@ -205,7 +251,7 @@ configure_sweep(doma_ctx * c)
   c->ntx = ntx;
   c->nty = nty;
   c->ntz = ntz;
-  c->ntasks = (int)(bricks.size() / 3) * c->ndirsets;
+  c->ntasks = (int)(bricks.size() / 3) * (c->ndirsets / c->ko);
   c->launches_per_sweep = c->schedule == 1 ? 1 : ndiag;
 
   // Brick-local cells ordered by hyperplane li+lj+lk.
@@ -222,7 +268,7 @@ configure_sweep(doma_ctx * c)
         const int li = p - lj - lk;
         if (li < 0 || li >= B)
           continue;
-        cells.push_back((uint32_t)li | ((uint32_t)lj << 8) | ((uint32_t)lk << 16));
+        cells.push_back((uint32_t)((lk * B + lj) * B + li));
       }
   }
   off[nplanes] = (int32_t)cells.size();
@@ -243,14 +289,14 @@ configure_sweep(doma_ctx * c)
   size_t nfx, nfy, nfz;
   if (c->dims == 3)
   {
-    nfx = (size_t)c->ndirsets * c->nyp * c->nzp * c->dl;
-    nfy = (size_t)c->ndirsets * c->nxp * c->nzp * c->dl;
-    nfz = (size_t)c->ndirsets * c->nxp * c->nyp * c->dl;
+    nfx = (size_t)c->ndirsets * c->nyp * c->nzp * c->dw;
+    nfy = (size_t)c->ndirsets * c->nxp * c->nzp * c->dw;
+    nfz = (size_t)c->ndirsets * c->nxp * c->nyp * c->dw;
   }
   else
   {
-    nfx = (size_t)c->ndirsets * c->nyp * c->dl;
-    nfy = (size_t)c->ndirsets * c->nxp * c->dl;
+    nfx = (size_t)c->ndirsets * c->nyp * c->dw;
+    nfy = (size_t)c->ndirsets * c->nxp * c->dw;
     nfz = 1;
   }
   c->face_bytes = (nfx + nfy + nfz) * rs;
@@ -267,7 +313,7 @@ configure_sweep(doma_ctx * c)
   if (c->schedule == 1)
   {
     int per_cu = 0;
-    if ((rc = flow_blocks_per_cu(c->dims, c->scheme, c->precision, c->exact, B, c->dl, T, &per_cu)))
+    if ((rc = sweep_blocks_per_cu(c->scheme, c->precision, c->exact, c->shape, 1, &per_cu)))
       return rc;
     hipDeviceProp_t prop;
     DOMA_HIP(hipGetDeviceProperties(&prop, c->device));
@@ -336,8 +382,8 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.wait_ticks = (long long)c->wall_clock_khz * 250;
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
-    int rc = launch_brick_sweep(a, c->dims, c->scheme, c->precision, c->exact, c->brick, c->dl,
-                                c->threads, 1, c->grid_flow, c->stream);
+    int rc = launch_brick_sweep(a, c->scheme, c->precision, c->exact, c->shape, 1, c->grid_flow,
+                                c->stream);
     if (rc)
       return rc;
     // Every work-group takes one ticket past the last task before it exits.
@@ -353,8 +399,8 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.diag_count = c->diag_off[s + 1] - c->diag_off[s];
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[2 * s], c->stream));
-    int rc = launch_brick_sweep(a, c->dims, c->scheme, c->precision, c->exact, c->brick, c->dl,
-                                c->threads, 0, 0, c->stream);
+    const int grid = a.diag_count * (c->ndirsets / c->ko);
+    int rc = launch_brick_sweep(a, c->scheme, c->precision, c->exact, c->shape, 0, grid, c->stream);
     if (rc)
       return rc;
     if (c->profiling)
@@ -724,6 +770,9 @@ doma_get_tuning(const doma_ctx * c, doma_tuning * t)
   t->threads = c->threads;
   t->exact_arith = c->exact;
   t->schedule = c->schedule;
+  t->octants = c->ko;
+  t->dirs_per_lane = c->ndl;
+  t->shape = c->shape + 1;
   return DOMA_OK;
 }
 
@@ -974,18 +1023,23 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
     return rc;
   if (g < 0 || g >= c->G)
     return fail(DOMA_ERR_INVALID, "doma_autotune: bad group");
+  // Every kernel shape valid for this problem, under both schedules.
   std::vector<doma_tuning> cands;
-  if (c->dims == 3)
-    for (int b : {4, 8})
-      for (int t : {128, 256, 512})
-        for (int sch : {0, 1})
-          cands.push_back(doma_tuning{b, t, 0, sch});
-  else if (c->dims == 2)
-    for (int b : {8, 16})
-      for (int sch : {0, 1})
-        cands.push_back(doma_tuning{b, 256, 0, sch});
+  if (c->dims == 1)
+    cands.push_back(doma_tuning{});
   else
-    cands.push_back(doma_tuning{0, 0, 0, 0});
+  {
+    const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
+    for (int s = 0; s < num_shapes(); ++s)
+      for (int sch : {0, 1})
+      {
+        doma_tuning t{};
+        t.shape = s + 1;
+        t.schedule = sch;
+        if (select_shape(c->dims, c->noct, c->chunk, minext, t) == s)
+          cands.push_back(t);
+      }
+  }
   const doma_tuning keep = c->tuning;
   doma_tuning win = keep;
   double win_ms = 1e300;

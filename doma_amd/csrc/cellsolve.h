@@ -62,18 +62,45 @@ twdd3d_exact(R q, R sig, R lx, R ly, R lz, R amu, R aeta, R axi, R & x, R & y, R
   return psi;
 }
 
+// Quotient n/d without the IEEE divide's scaling and fix-up steps, for operands
+// that are normal and finite (all sweep quantities are): FP64 refines the
+// hardware reciprocal with two Newton steps and one residual correction
+// (within about one ulp); FP32 uses the hardware reciprocal directly (about two
+// ulp, inside the FP32 sweep's error budget).
+__device__ __forceinline__ double
+fast_div(double n, double d)
+{
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  const double q = n * r;
+  return fma(r, fma(-d, q, n), q);
+}
+
+__device__ __forceinline__ float
+fast_div(float n, float d)
+{
+  return n * __builtin_amdgcn_rcpf(d);
+}
+
 // Inverse TWDD weight 1/a for a = 1 - n/d clamped to [0.5, 1]: a < 0.5 <=> n > d/2
-// and a > 1 <=> n < 0 (d > 0 whenever the upwind flux exceeds 1e-8).
+// and a > 1 <=> n < 0 (d > 0 whenever the upwind flux exceeds 1e-8).  Branch
+// free: an infinite or NaN quotient (d - n == 0) is always replaced by a clamp.
 template <typename R>
 __device__ __forceinline__ R
 inv_weight(R n, R d)
 {
-  R r = d / (d - n);
+  R r = fast_div(d, d - n);
   r = n < R(0) ? R(1) : r;
   r = n >= R(0.5) * d ? R(2) : r;
   return r;
 }
 
+// Straight-line TWDD3D: the three weights are always formed and the reference's
+// "upwind flux > 1e-8" test selects between them and the 0.5 default, so a lane
+// can interleave the solves of its directions.
 template <typename R>
 __device__ __forceinline__ R
 twdd3d_fast(R qV, R sV, R mA, R eB, R xC, R & x, R & y, R & z)
@@ -81,15 +108,15 @@ twdd3d_fast(R qV, R sV, R mA, R eB, R xC, R & x, R & y, R & z)
   const R th = R(0.9);
   const R qVs = qV * th;
   const R mAx = mA * x, eBy = eB * y, xCz = xC * z;
-  R ia = R(2), ib = R(2), ic = R(2);
-  if (x > R(1e-8))
-    ia = inv_weight(qVs + (eBy + xCz) * th + mAx, (sV + R(2) * (eB + xC)) * x);
-  if (y > R(1e-8))
-    ib = inv_weight(qVs + (mAx + xCz) * th + eBy, (sV + R(2) * (mA + xC)) * y);
-  if (z > R(1e-8))
-    ic = inv_weight(qVs + (mAx + eBy) * th + xCz, (sV + R(2) * (mA + eB)) * z);
+  const R two_s = sV + R(2) * (mA + eB + xC);
+  R ia = inv_weight(qVs + (eBy + xCz) * th + mAx, (two_s - R(2) * mA) * x);
+  R ib = inv_weight(qVs + (mAx + xCz) * th + eBy, (two_s - R(2) * eB) * y);
+  R ic = inv_weight(qVs + (mAx + eBy) * th + xCz, (two_s - R(2) * xC) * z);
+  ia = x > R(1e-8) ? ia : R(2);
+  ib = y > R(1e-8) ? ib : R(2);
+  ic = z > R(1e-8) ? ic : R(2);
   const R wa = mA * ia, wb = eB * ib, wc = xC * ic;
-  const R psi = (qV + wa * x + wb * y + wc * z) / (sV + wa + wb + wc);
+  const R psi = fast_div(qV + wa * x + wb * y + wc * z, sV + wa + wb + wc);
   x = ia * psi - (ia - R(1)) * x;
   y = ib * psi - (ib - R(1)) * y;
   z = ic * psi - (ic - R(1)) * z;
@@ -113,7 +140,7 @@ template <typename R>
 __device__ __forceinline__ R
 dd3d_fast(R q, R sig, R cx, R cy, R cz, R & x, R & y, R & z)
 {
-  const R psi = (q + cx * x + cy * y + cz * z) / (cx + cy + cz + sig);
+  const R psi = fast_div(q + cx * x + cy * y + cz * z, cx + cy + cz + sig);
   x = R(2) * psi - x;
   y = R(2) * psi - y;
   z = R(2) * psi - z;
@@ -161,13 +188,12 @@ twdd2d_fast(R qV, R sV, R mA, R eB, R axi, R & x, R & y)
   const R th = R(0.9);
   const R qVs = qV * th;
   const R mAx = mA * x, eBy = eB * y;
-  R ia = R(2), ib = R(2);
-  if (x > R(1e-8))
-    ia = inv_weight(qVs + eBy * th + mAx, (sV + R(2) * eB) * x);
-  if (y > R(1e-8))
-    ib = inv_weight(qVs + (mAx + axi) * th + eBy, (sV + R(2) * mA) * y);
+  R ia = inv_weight(qVs + eBy * th + mAx, (sV + R(2) * eB) * x);
+  R ib = inv_weight(qVs + (mAx + axi) * th + eBy, (sV + R(2) * mA) * y);
+  ia = x > R(1e-8) ? ia : R(2);
+  ib = y > R(1e-8) ? ib : R(2);
   const R wa = mA * ia, wb = eB * ib;
-  const R psi = (qV + wa * x + wb * y) / (sV + wa + wb);
+  const R psi = fast_div(qV + wa * x + wb * y, sV + wa + wb);
   x = ia * psi - (ia - R(1)) * x;
   y = ib * psi - (ib - R(1)) * y;
   return psi;

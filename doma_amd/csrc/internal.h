@@ -74,13 +74,22 @@ struct Sweep1DArgs
   double * part;          // [2*dirs][nx]
 };
 
-// Launch wrappers implemented in sweep.hip.
-int launch_brick_sweep(const BrickArgs & a, int dims, int scheme, int precision, int exact,
-                       int brick, int dl, int threads, int schedule, int grid_flow, hipStream_t s);
-int flow_blocks_per_cu(int dims, int scheme, int precision, int exact, int brick, int dl,
-                       int threads, int * per_cu);
+// Kernel shapes of the brick sweep (table in sweep_impl.h).
+struct ShapeInfo
+{
+  int dims, brick, dl, ndl, ko, threads;
+};
+bool shape_info(int shape, ShapeInfo * out);
+int num_shapes();
+size_t shape_lds_bytes(int shape, int precision);
+
+// Launch wrappers implemented in sweep.hip.  schedule 0: one launch per brick
+// diagonal (grid = work-groups of this diagonal); 1: persistent dataflow launch.
+int launch_brick_sweep(const BrickArgs & a, int scheme, int precision, int exact, int shape,
+                       int schedule, int grid, hipStream_t s);
+int sweep_blocks_per_cu(int scheme, int precision, int exact, int shape, int schedule,
+                        int * per_cu);
 int launch_sweep1d(const Sweep1DArgs & a, int precision, hipStream_t s);
-bool brick_variant_exists(int dims, int brick, int dl, int threads);
 
 // Launch wrappers implemented in iterate.hip.
 struct IterArgs
@@ -164,7 +173,8 @@ struct doma_ctx
 
   // sweep decomposition
   doma_tuning tuning{};
-  int brick = 0, dl = 0, threads = 0, exact = 0;
+  int shape = -1;  // kernel shape id (sweep_impl.h)
+  int brick = 0, dl = 0, threads = 0, exact = 0, ndl = 1, ko = 1, dw = 0;
   int chunk = 0, nchunks = 0, ndirsets = 0;
   int nxp = 0, nyp = 0, nzp = 0;
   std::vector<int> diag_off;  // [ndiag+1]
@@ -197,6 +207,7 @@ struct doma_ctx
 
 namespace doma
 {
+int select_shape(int dims, int noct, int chunk, int minext, const doma_tuning & t);
 int configure_sweep(doma_ctx * c);
 int run_sweep(doma_ctx * c, int g);
 int check_sweep_error(doma_ctx * c);

@@ -1,0 +1,626 @@
+// sweep_impl.h -- brick-wavefront transport sweep for CDNA4 (gfx950): kernel
+// bodies.  Instantiated by sweep_*.hip; launched through sweep.hip.
+//
+// Decomposition (2D and 3D):
+//   * The grid is cut into B^3 bricks (B^2 tiles in 2D).  Every octant visits
+//     the bricks in its own sweep orientation; in oriented coordinates a brick
+//     depends on its upwind neighbours along x, y and z only.  Vacuum boundaries
+//     make the octants independent (TransportSolver3D.cpp:565-607,
+//     CartesianCell3D.cpp:31-38), so all octants are swept at once.
+//   * A work-group owns one oriented brick for KO octants and one direction
+//     chunk at a time.  The KO octants see the same brick-local plane structure
+//     (li+lj+lk = p in oriented coordinates), so their cells are packed into the
+//     same passes; cells of a plane are independent and run concurrently.
+//   * Threads are (octant, cell slot, direction lane).  DL consecutive lanes
+//     carry NDL directions each, so the per-cell source and cross section are
+//     uniform per lane group and a lane has NDL independent cell solves per pass.
+//   * Everything a brick needs is staged in LDS before its first plane: upwind
+//     face fluxes (B^2 x-, y- and z-faces per direction, updated in place), the
+//     cell source and cross-section products, the global cell index and the
+//     widths.  The plane loop touches no global memory except one store of the
+//     direction-reduced scalar flux per cell; only the brick's inflow/outflow
+//     faces cross HBM, through per-direction-set face planes.
+//   * The scalar flux of a cell is reduced over its DL lanes with a fixed DPP
+//     butterfly and stored once per (cell, direction set); the direction sets
+//     are summed in a fixed order afterwards, so results are reproducible run to
+//     run (no floating-point atomics).
+//
+// Two schedules share the brick body:
+//   * diagonal: one launch per brick diagonal bi+bj+bk = s;
+//   * dataflow: one persistent launch per sweep.  Work-groups take (brick,
+//     octant group, chunk) tasks from a monotone ticket counter in diagonal
+//     order and wait on the epoch flags of the (at most three) upwind tasks.  A
+//     task only waits on smaller tickets, taken by work-groups that are already
+//     running, so the schedule cannot deadlock; every wait is bounded by a
+//     wall-clock deadline and reports through an error flag.  Face planes are
+//     written and read with device-coherent accesses, so no cache-wide
+//     write-back or invalidate is needed per task.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cellsolve.h"
+#include "internal.h"
+
+namespace doma
+{
+namespace sweepk
+{
+// Kernel shapes: brick edge, direction lanes per cell, directions per lane,
+// octants per work-group, threads per work-group.
+struct Shape
+{
+  int dims, B, DL, NDL, KO, NT;
+};
+
+// Shape table (index = shape id).  Chosen for direction chunks of <= 16, <= 32
+// and <= 64 per octant, small and large bricks, in 3D and 2D.
+constexpr Shape kShapes[] = {
+    {3, 4, 8, 2, 4, 128},   // 0: <=16 dirs, 4^3 bricks, 4 octants
+    {3, 4, 8, 2, 8, 256},   // 1: <=16 dirs, 4^3 bricks, 8 octants
+    {3, 8, 8, 2, 2, 256},   // 2: <=16 dirs, 8^3 bricks, 2 octants
+    {3, 4, 16, 2, 4, 256},  // 3: <=32 dirs, 4^3 bricks, 4 octants
+    {3, 8, 16, 2, 1, 256},  // 4: <=32 dirs, 8^3 bricks
+    {3, 8, 16, 2, 2, 512},  // 5: <=32 dirs, 8^3 bricks, 2 octants
+    {3, 4, 32, 2, 2, 256},  // 6: <=64 dirs, 4^3 bricks, 2 octants
+    {3, 8, 32, 2, 1, 256},  // 7: <=64 dirs, 8^3 bricks
+    {3, 4, 8, 1, 4, 128},   // 8: <=8 dirs, 4^3 bricks, 4 octants
+    {3, 8, 32, 1, 1, 512},  // 9: <=32 dirs, one direction per lane
+    {2, 8, 8, 2, 4, 128},   // 10: 2D <=16 dirs, 8^2 tiles, 4 quadrants
+    {2, 16, 16, 2, 2, 256}, // 11: 2D <=32 dirs, 16^2 tiles, 2 quadrants
+    {2, 8, 32, 2, 1, 256},  // 12: 2D <=64 dirs, 8^2 tiles
+    {2, 8, 8, 1, 4, 128},   // 13: 2D <=8 dirs
+    {3, 4, 16, 1, 1, 256},  // 14: <=16 dirs, 4^3 bricks, one direction per lane
+    {3, 4, 16, 1, 1, 128},  // 15: <=16 dirs, 4^3 bricks, 8 cells per pass
+    {3, 8, 16, 1, 1, 512},  // 16: <=16 dirs, 8^3 bricks
+    {3, 4, 32, 1, 1, 256},  // 17: <=32 dirs, 4^3 bricks
+    {3, 8, 64, 1, 1, 512},  // 18: <=64 dirs, 8^3 bricks
+    {2, 8, 16, 1, 1, 256},  // 19: 2D <=16 dirs
+    {2, 16, 32, 1, 1, 512}, // 20: 2D <=32 dirs, 16^2 tiles
+};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+// ------------------------------------------------------------------------- //
+// Direction reduction                                                        //
+// ------------------------------------------------------------------------- //
+
+template <int CTRL>
+__device__ __forceinline__ double
+dpp_mov(double v)
+{
+  const int lo = __double2loint(v);
+  const int hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(rhi, rlo);
+}
+
+// DPP controls (GFX9 encoding): quad_perm(1,0,3,2), quad_perm(2,3,0,1),
+// row_half_mirror, row_mirror.
+constexpr int DPP_QP_XOR1 = 0xB1;
+constexpr int DPP_QP_XOR2 = 0x4E;
+constexpr int DPP_HALF_MIRROR = 0x141;
+constexpr int DPP_MIRROR = 0x140;
+
+// Fixed-order sum over groups of DL (8..64) consecutive lanes; every lane of a
+// group ends with the same bitwise result (each step adds a commutative pair).
+template <int DL>
+__device__ __forceinline__ double
+group_sum(double v)
+{
+  static_assert(DL >= 8 && DL <= 64, "direction lanes");
+  v += dpp_mov<DPP_QP_XOR1>(v);
+  v += dpp_mov<DPP_QP_XOR2>(v);
+  v += dpp_mov<DPP_HALF_MIRROR>(v);
+  if constexpr (DL >= 16)
+    v += dpp_mov<DPP_MIRROR>(v);
+  if constexpr (DL >= 32)
+    v += __shfl_xor(v, 16, 32);
+  if constexpr (DL >= 64)
+    v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------------- //
+// LDS layout                                                                 //
+// ------------------------------------------------------------------------- //
+
+template <typename R, int DIM, int B, int DL, int NDL, int KO>
+struct Layout
+{
+  static constexpr int DW = DL * NDL;                      // direction slots per face slot
+  static constexpr int FS = DIM == 3 ? B * B : B;          // face slots per axis
+  static constexpr int NC = DIM == 3 ? B * B * B : B * B;  // cells per brick
+  static constexpr int NPLANES = DIM == 3 ? 3 * B - 2 : 2 * B - 1;
+  static constexpr int NAX = DIM == 3 ? 3 : 2;
+  static constexpr size_t FACE = sizeof(R) * FS * DW;      // one axis, one octant
+  static constexpr size_t FACES_OFF = 0;
+  static constexpr size_t CELL_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;  // double2 [KO][NC]
+  static constexpr size_t GIDX_OFF = CELL_OFF + 16 * NC * KO;             // int [KO][NC]
+  static constexpr size_t GEO_OFF = GIDX_OFF + 4 * NC * KO;               // double [KO][3][B]
+  static constexpr size_t PT_OFF = GEO_OFF + 8 * 3 * B * KO;              // uint16 [NC]
+  static constexpr size_t PO_OFF = (PT_OFF + 2 * NC + 3) / 4 * 4;         // int [NPLANES+1]
+  static constexpr size_t TASK_OFF = PO_OFF + 4 * (NPLANES + 1);
+  static constexpr size_t BYTES = TASK_OFF + 16;
+};
+
+template <typename R, int DIM, int B, int DL, int NDL, int KO, int NT>
+__device__ __forceinline__ void
+load_plane_table(const BrickArgs & a, unsigned char * smem)
+{
+  using L = Layout<R, DIM, B, DL, NDL, KO>;
+  uint16_t * pt = reinterpret_cast<uint16_t *>(smem + L::PT_OFF);
+  int32_t * po = reinterpret_cast<int32_t *>(smem + L::PO_OFF);
+  for (int e = threadIdx.x; e < L::NC; e += NT)
+    pt[e] = static_cast<uint16_t>(a.plane_cells[e]);
+  for (int e = threadIdx.x; e <= L::NPLANES; e += NT)
+    po[e] = a.plane_off[e];
+}
+
+// Orientation of an octant (3D: 4=-mu 2=-eta 1=-xi, TransportBase.h:22-32) or a
+// 2D quadrant slot (PPP, PMP, MPP, MMP: 2=-mu 1=-eta).
+template <int DIM>
+__device__ __forceinline__ void
+orient(int oct, bool & nx, bool & ny, bool & nz)
+{
+  nx = DIM == 3 ? (oct & 4) : (oct & 2);
+  ny = DIM == 3 ? (oct & 2) : (oct & 1);
+  nz = DIM == 3 ? (oct & 1) : false;
+}
+
+// Device-coherent face-plane accesses for the dataflow schedule (bypass the
+// per-XCD L2 so that a consumer on another XCD sees the producer's stores once
+// they have completed); plain accesses for the diagonal schedule, where the
+// kernel boundary orders producer and consumer.
+template <bool COHERENT, typename R>
+__device__ __forceinline__ R
+face_load(const R * p)
+{
+  if constexpr (COHERENT)
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    return *p;
+}
+
+template <bool COHERENT, typename R>
+__device__ __forceinline__ void
+face_store(R * p, R v)
+{
+  if constexpr (COHERENT)
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    *p = v;
+}
+
+// ------------------------------------------------------------------------- //
+// Brick body                                                                 //
+// ------------------------------------------------------------------------- //
+
+template <typename R, int DIM, int SCHEME, bool EXACT, int B, int DL, int NDL, int KO, int NT,
+          bool COHERENT>
+__device__ __forceinline__ void
+sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigned char * smem)
+{
+  using L = Layout<R, DIM, B, DL, NDL, KO>;
+  constexpr int DW = L::DW, FS = L::FS, NC = L::NC, NAX = L::NAX;
+  constexpr int CPP = NT / DL;   // cell slots per pass
+  constexpr int CPO = CPP / KO;  // cells per octant per pass
+  static_assert(CPP % KO == 0, "cell slots must split evenly over octants");
+
+  R * faces = reinterpret_cast<R *>(smem + L::FACES_OFF);  // [KO][NAX][FS][DW]
+  double2 * cd = reinterpret_cast<double2 *>(smem + L::CELL_OFF);
+  int * gidx = reinterpret_cast<int *>(smem + L::GIDX_OFF);
+  double * geo = reinterpret_cast<double *>(smem + L::GEO_OFF);
+  const uint16_t * pt = reinterpret_cast<const uint16_t *>(smem + L::PT_OFF);
+  const int32_t * po = reinterpret_cast<const int32_t *>(smem + L::PO_OFF);
+
+  const int tid = threadIdx.x;
+  const int dd = tid % DL;
+  const int slot = tid / DL;
+  const int ko = slot % KO;
+  const int cpo = slot / KO;
+  const int oct = og * KO + ko;
+  const int ds = oct * a.nchunks + ch;
+
+  R amu[NDL], aeta[NDL], axi[NDL];
+  double w[NDL];
+#pragma unroll
+  for (int m = 0; m < NDL; ++m)
+  {
+    const int lane_dir = dd * NDL + m;
+    const int n = ch * a.chunk + lane_dir;
+    amu[m] = aeta[m] = axi[m] = R(1);
+    w[m] = 0.0;
+    if (lane_dir < a.chunk && n < a.dirs)
+    {
+      const double * t = a.dirtab + 4 * (oct * a.dirs + n);
+      amu[m] = R(t[0]);
+      aeta[m] = R(t[1]);
+      axi[m] = R(t[2]);
+      w[m] = t[3];
+    }
+  }
+
+  const int i0 = bi * B, j0 = bj * B, k0 = bk * B;
+  R * gf[3] = {reinterpret_cast<R *>(a.fx), reinterpret_cast<R *>(a.fy),
+               reinterpret_cast<R *>(a.fz)};
+
+  // The previous task of this work-group may still be reading LDS.
+  __syncthreads();
+
+  // Inflow faces of every octant: the upwind brick's outflow, or vacuum.
+  for (int e = tid; e < KO * NAX * FS * DW; e += NT)
+  {
+    const int k_o = e / (NAX * FS * DW);
+    const int r = e - k_o * (NAX * FS * DW);
+    const int ax = r / (FS * DW);
+    const int s = (r / DW) % FS, l = r % DW;
+    const int u = DIM == 3 ? s / B : s;
+    const int v = DIM == 3 ? s % B : 0;
+    const int dsk = (og * KO + k_o) * a.nchunks + ch;
+    R val = R(0);
+    if (ax == 0 && bi > 0)
+      val = face_load<COHERENT>(gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp +
+                                                     (size_t)(j0 + u) * a.nzp + (k0 + v))
+                                                  : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l);
+    else if (ax == 1 && bj > 0)
+      val = face_load<COHERENT>(gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp +
+                                                     (size_t)(i0 + u) * a.nzp + (k0 + v))
+                                                  : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l);
+    else if (ax == 2 && bk > 0)
+      val = face_load<COHERENT>(gf[2] + ((size_t)dsk * a.nxp * a.nyp + (size_t)(i0 + u) * a.nyp +
+                                         (j0 + v)) * DW + l);
+    faces[e] = val;
+  }
+  // Cell data of every octant, brick-local order (lk*B + lj)*B + li.
+  for (int e = tid; e < KO * NC; e += NT)
+  {
+    const int k_o = e / NC, ce = e - k_o * NC;
+    bool nx, ny, nz;
+    orient<DIM>(og * KO + k_o, nx, ny, nz);
+    const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
+    const int oi = i0 + li, oj = j0 + lj, ok = k0 + lk;
+    double2 v = make_double2(0.0, 1.0);
+    int c = -1;
+    if (oi < a.nx && oj < a.ny && ok < a.nz)
+    {
+      const int i = nx ? a.nx - 1 - oi : oi;
+      const int j = ny ? a.ny - 1 - oj : oj;
+      const int k = nz ? a.nz - 1 - ok : ok;
+      c = (k * a.ny + j) * a.nx + i;
+      const double q = a.q[c], sg = a.sig[c];
+      if (EXACT || SCHEME == DOMA_SCHEME_DD)
+        v = make_double2(q, sg);
+      else
+      {
+        const double V = DIM == 3 ? a.lx[i] * a.ly[j] * a.lz[k] : a.lx[i] * a.ly[j];
+        v = make_double2(q * V, sg * V);
+      }
+    }
+    cd[e] = v;
+    gidx[e] = c;
+  }
+  for (int e = tid; e < KO * 3 * B; e += NT)
+  {
+    const int k_o = e / (3 * B), r = e - k_o * 3 * B;
+    const int ax = r / B, l = r - ax * B;
+    bool nx, ny, nz;
+    orient<DIM>(og * KO + k_o, nx, ny, nz);
+    double v = 1.0;
+    if (ax == 0 && i0 + l < a.nx)
+      v = a.lx[nx ? a.nx - 1 - (i0 + l) : i0 + l];
+    else if (ax == 1 && j0 + l < a.ny)
+      v = a.ly[ny ? a.ny - 1 - (j0 + l) : j0 + l];
+    else if (ax == 2 && DIM == 3 && k0 + l < a.nz)
+      v = a.lz[nz ? a.nz - 1 - (k0 + l) : k0 + l];
+    geo[e] = v;
+  }
+  __syncthreads();
+
+  R * fxo = faces + (size_t)ko * NAX * FS * DW;  // this lane's octant
+  R * fyo = fxo + FS * DW;
+  R * fzo = fyo + FS * DW;
+  const double2 * cdo = cd + ko * NC;
+  const int * gio = gidx + ko * NC;
+  const double * gxo = geo + ko * 3 * B;
+
+  for (int p = 0; p < L::NPLANES; ++p)
+  {
+    const int pe = po[p + 1];
+    for (int pos = po[p] + cpo; pos < pe; pos += CPO)
+    {
+      const int ce = pt[pos];
+      const int c = gio[ce];
+      if (c < 0)
+        continue;  // outside a partial brick: the whole lane group skips
+      const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
+      const double2 cv = cdo[ce];
+      const double lxd = gxo[li], lyd = gxo[B + lj];
+      const double lzd = DIM == 3 ? gxo[2 * B + lk] : 1.0;
+      R * xf = fxo + (DIM == 3 ? (lj * B + lk) : lj) * DW + dd * NDL;
+      R * yf = fyo + (DIM == 3 ? (li * B + lk) : li) * DW + dd * NDL;
+      R * zf = fzo + (li * B + lj) * DW + dd * NDL;
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < NDL; ++m)
+      {
+        R x = xf[m], y = yf[m];
+        R z = DIM == 3 ? zf[m] : R(0);
+        R psi;
+        if constexpr (DIM == 3)
+        {
+          if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
+          {
+            if constexpr (EXACT)
+              psi = twdd3d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), R(lzd), amu[m], aeta[m],
+                                    axi[m], x, y, z);
+            else
+              psi = twdd3d_fast<R>(R(cv.x), R(cv.y), amu[m] * R(lyd * lzd),
+                                   aeta[m] * R(lxd * lzd), axi[m] * R(lxd * lyd), x, y, z);
+          }
+          else
+          {
+            if constexpr (EXACT)
+              psi = dd3d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), R(lzd), amu[m], aeta[m],
+                                  axi[m], x, y, z);
+            else
+              psi = dd3d_fast<R>(R(cv.x), R(cv.y), R(2) * amu[m] * R(1.0 / lxd),
+                                 R(2) * aeta[m] * R(1.0 / lyd), R(2) * axi[m] * R(1.0 / lzd), x, y,
+                                 z);
+          }
+        }
+        else
+        {
+          if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
+          {
+            if constexpr (EXACT)
+              psi = twdd2d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), amu[m], aeta[m], axi[m], x, y);
+            else
+              psi = twdd2d_fast<R>(R(cv.x), R(cv.y), amu[m] * R(lyd), aeta[m] * R(lxd), axi[m], x, y);
+          }
+          else
+          {
+            if constexpr (EXACT)
+              psi = dd2d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), amu[m], aeta[m], x, y);
+            else
+              psi = dd3d_fast<R>(R(cv.x), R(cv.y), R(2) * amu[m] * R(1.0 / lxd),
+                                 R(2) * aeta[m] * R(1.0 / lyd), R(0), x, y, z);
+          }
+        }
+        xf[m] = x;
+        yf[m] = y;
+        if constexpr (DIM == 3)
+          zf[m] = z;
+        s += w[m] * static_cast<double>(psi);
+      }
+      s = group_sum<DL>(s);
+      if (dd == 0)
+        a.part[(size_t)ds * a.N + c] = s;
+    }
+    __syncthreads();
+  }
+
+  // Outflow faces for the downwind bricks (padded extents keep every index in range).
+  for (int e = tid; e < KO * NAX * FS * DW; e += NT)
+  {
+    const int k_o = e / (NAX * FS * DW);
+    const int r = e - k_o * (NAX * FS * DW);
+    const int ax = r / (FS * DW);
+    const int s = (r / DW) % FS, l = r % DW;
+    const int u = DIM == 3 ? s / B : s;
+    const int v = DIM == 3 ? s % B : 0;
+    const int dsk = (og * KO + k_o) * a.nchunks + ch;
+    R * dst;
+    if (ax == 0)
+      dst = gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp + (size_t)(j0 + u) * a.nzp + (k0 + v))
+                              : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l;
+    else if (ax == 1)
+      dst = gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp + (size_t)(i0 + u) * a.nzp + (k0 + v))
+                              : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l;
+    else
+      dst = gf[2] + ((size_t)dsk * a.nxp * a.nyp + (size_t)(i0 + u) * a.nyp + (j0 + v)) * DW + l;
+    face_store<COHERENT>(dst, faces[e]);
+  }
+}
+
+// One launch per brick diagonal: block = (brick on the diagonal, octant group, chunk).
+template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
+__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+brick_sweep_kernel(BrickArgs a)
+{
+  constexpr Shape S = kShapes[SHAPE];
+  extern __shared__ __align__(16) unsigned char smem[];
+  load_plane_table<R, DIM, S.B, S.DL, S.NDL, S.KO, S.NT>(a, smem);
+  const int ngroups = a.ndirsets / (a.nchunks * S.KO);  // octant groups
+  const int per_brick = ngroups * a.nchunks;
+  const int br = a.diag_begin + blockIdx.x / per_brick;
+  const int r = blockIdx.x % per_brick;
+  const int og = r / a.nchunks, ch = r % a.nchunks;
+  sweep_brick<R, DIM, SCHEME, EXACT, S.B, S.DL, S.NDL, S.KO, S.NT, false>(
+      a, a.bricks[3 * br + 0], a.bricks[3 * br + 1], a.bricks[3 * br + 2], og, ch, smem);
+}
+
+// Bounded wait for an upwind task's epoch flag (thread 0 only).  Polls with
+// relaxed device-coherent loads; gives up at a wall-clock deadline, or at once
+// when another wait of this sweep has already expired.
+__device__ __forceinline__ void
+wait_flag(const BrickArgs & a, const uint32_t * f)
+{
+  if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch)
+    return;
+  const long long t0 = wall_clock64();
+  int polls = 0;
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.epoch)
+  {
+    __builtin_amdgcn_s_sleep(2);
+    if ((++polls & 31) == 0)
+    {
+      if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return;
+      if (wall_clock64() - t0 > a.wait_ticks)
+      {
+        atomicExch(a.error, 1);
+        return;
+      }
+    }
+  }
+}
+
+// One persistent launch per sweep; tasks in ticket (= diagonal) order, task id
+// = (brick, octant group, chunk).
+template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
+__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+brick_flow_kernel(BrickArgs a)
+{
+  constexpr Shape S = kShapes[SHAPE];
+  using L = Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int * s_task = reinterpret_cast<int *>(smem + L::TASK_OFF);
+  load_plane_table<R, DIM, S.B, S.DL, S.NDL, S.KO, S.NT>(a, smem);
+  const int nbricks = a.ntx * a.nty * a.ntz;
+  const int per_brick = a.ndirsets / S.KO;  // (octant group, chunk) pairs
+  while (true)
+  {
+    // A barrier heads every iteration so that the divergent "publish flag" of
+    // one task and "take ticket" of the next can never be merged across the
+    // back-edge into a nested loop that lane 0 leaves without its wave.
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      const unsigned long long t = atomicAdd(a.ticket, 1ull) - a.ticket_base;
+      *s_task = t < (unsigned long long)a.ntasks ? (int)t : -1;
+    }
+    __syncthreads();
+    // Uniform by construction; readfirstlane lets the compiler see it.
+    const int t = __builtin_amdgcn_readfirstlane(*s_task);
+    if (t < 0)
+      break;
+    const int br = t / per_brick;
+    const int r = t - br * per_brick;
+    const int og = r / a.nchunks, ch = r % a.nchunks;
+    const int bi = a.bricks[3 * br + 0], bj = a.bricks[3 * br + 1], bk = a.bricks[3 * br + 2];
+    uint32_t * flags = a.flags + (size_t)r * nbricks;
+    if (threadIdx.x == 0)
+    {
+      if (bi > 0)
+        wait_flag(a, flags + (bk * a.nty + bj) * a.ntx + (bi - 1));
+      if (bj > 0)
+        wait_flag(a, flags + (bk * a.nty + (bj - 1)) * a.ntx + bi);
+      if (bk > 0)
+        wait_flag(a, flags + ((bk - 1) * a.nty + bj) * a.ntx + bi);
+    }
+    __syncthreads();
+    sweep_brick<R, DIM, SCHEME, EXACT, S.B, S.DL, S.NDL, S.KO, S.NT, true>(a, bi, bj, bk, og, ch,
+                                                                           smem);
+    // Every thread's coherent face stores must have completed before the flag.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename K>
+int
+prepare(K k, size_t shmem, bool & done)
+{
+  if (done)
+    return DOMA_OK;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  if (e != hipSuccess)
+    return hip_fail(e, "hipFuncSetAttribute(sweep kernel)");
+  done = true;
+  return DOMA_OK;
+}
+
+template <typename R, int DIM, int SHAPE>
+constexpr size_t
+shmem_bytes()
+{
+  constexpr Shape S = kShapes[SHAPE];
+  return Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>::BYTES;
+}
+
+// Launch (op 0) or occupancy query (op 1) of one instantiated kernel.
+template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
+int
+run_shape(const BrickArgs & a, int op, int schedule, int grid, hipStream_t s, int * per_cu)
+{
+  constexpr Shape S = kShapes[SHAPE];
+  constexpr size_t shmem = shmem_bytes<R, DIM, SHAPE>();
+  if (schedule == 0)
+  {
+    auto k = brick_sweep_kernel<R, DIM, SCHEME, EXACT, SHAPE>;
+    static bool ready = false;
+    int rc = prepare(k, shmem, ready);
+    if (rc)
+      return rc;
+    if (op == 1)
+    {
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
+      return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
+  }
+  else
+  {
+    auto k = brick_flow_kernel<R, DIM, SCHEME, EXACT, SHAPE>;
+    static bool ready = false;
+    int rc = prepare(k, shmem, ready);
+    if (rc)
+      return rc;
+    if (op == 1)
+    {
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
+      return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
+  }
+  return DOMA_OK;
+}
+
+// Per-(precision, scheme) dispatch over the shape table; defined once per
+// instantiation unit.
+template <typename R, int SCHEME>
+int dispatch_shapes(const BrickArgs & a, int exact, int shape, int op, int schedule, int grid,
+                    hipStream_t s, int * per_cu);
+
+template <typename R, int SCHEME, bool EXACT, int... IDS>
+int
+dispatch_seq(const BrickArgs & a, int shape, int op, int schedule, int grid, hipStream_t s,
+             int * per_cu, std::integer_sequence<int, IDS...>)
+{
+  int rc = DOMA_ERR_INVALID;
+  bool hit = false;
+  (void)((shape == IDS
+              ? (hit = true,
+                 rc = run_shape<R, kShapes[IDS].dims, SCHEME, EXACT, IDS>(a, op, schedule, grid, s,
+                                                                          per_cu),
+                 true)
+              : false) ||
+         ...);
+  if (!hit)
+    return fail(DOMA_ERR_INVALID, "unknown sweep kernel shape");
+  return rc;
+}
+} // namespace sweepk
+} // namespace doma
+
+// Defines dispatch_shapes<R, SCHEME> for one (precision, scheme) pair.
+#define DOMA_SWEEP_INSTANTIATE(R, SCHEME)                                                          \
+  namespace doma                                                                                   \
+  {                                                                                                \
+  namespace sweepk                                                                                 \
+  {                                                                                                \
+  template <>                                                                                      \
+  int dispatch_shapes<R, SCHEME>(const BrickArgs & a, int exact, int shape, int op, int schedule, \
+                                 int grid, hipStream_t s, int * per_cu)                            \
+  {                                                                                                \
+    using seq = std::make_integer_sequence<int, kNumShapes>;                                       \
+    return exact ? dispatch_seq<R, SCHEME, true>(a, shape, op, schedule, grid, s, per_cu, seq{})   \
+                 : dispatch_seq<R, SCHEME, false>(a, shape, op, schedule, grid, s, per_cu, seq{}); \
+  }                                                                                                \
+  }                                                                                                \
+  }

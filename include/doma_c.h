@@ -65,14 +65,20 @@ typedef struct doma_problem_desc {
   const double *mu, *eta, *xi, *w;
 } doma_problem_desc;
 
-/* Kernel shape.  Zero fields mean "library default" (or the autotuned choice
- * when doma_autotune has been run on this context). */
+/* Kernel shape of the sweep.  Zero fields mean "any" (the library default, or
+ * the autotuned choice once doma_autotune has run for this problem shape);
+ * non-zero fields restrict the choice.  doma_get_tuning reports the shape in use
+ * with every field filled in. */
 typedef struct doma_tuning {
-  int32_t brick;         /* 3D: brick edge in cells (4 or 8); 2D: tile edge (8 or 16) */
-  int32_t threads;       /* work-group size of the sweep kernel (64, 128, 256 or 512) */
-  int32_t exact_arith;   /* 1: reference expression order with IEEE divides,
-                            0: hoisted reciprocals (default) */
-  int32_t schedule;      /* 0: one launch per brick diagonal (default) */
+  int32_t brick;          /* 3D: brick edge in cells (4 or 8); 2D: tile edge (8 or 16) */
+  int32_t threads;        /* work-group size of the sweep kernel */
+  int32_t exact_arith;    /* 1: reference expression order with IEEE divides,
+                             0: hoisted reciprocals (default) */
+  int32_t schedule;       /* 0: one launch per brick diagonal (default);
+                             1: one persistent dataflow launch per sweep */
+  int32_t octants;        /* octants (2D: quadrants) per work-group: 1, 2, 4 or 8 */
+  int32_t dirs_per_lane;  /* directions per lane: 1 or 2 */
+  int32_t shape;          /* explicit kernel shape id + 1 (overrides the hints) */
 } doma_tuning;
 
 typedef struct doma_stats {
