@@ -62,6 +62,43 @@ def cpu_baseline(problem, seconds: float, threads: int):
                       f"with per-thread partials, {threads} threads"}
 
 
+def sweeps_only(args):
+    """Profiling mode: a fixed kernel shape, n sweeps of group 0 of the benchmark
+    problem (plus one warm-up), nothing else on the GPU."""
+    import numpy as np
+
+    from doma_amd import workloads
+    from doma_amd.capi import FP32_SWEEP, FP64, Context
+
+    if args.workload == "multigroup_full":
+        p = workloads.multigroup_full(args.n, 8, 8, 4)
+    else:
+        p = workloads.random_blocks(args.n, 3, 4, 4)
+    ctx = Context(p, precision=FP32_SWEEP if args.fp32 else FP64)
+    ctx.set_tuning(shape=args.shape, schedule=args.schedule, exact_arith=int(args.exact))
+    q = np.random.default_rng(0).uniform(0.0, 1.0, p.N) / (4 * np.pi)
+    ctx.sweep(0, q)
+    ms, launches = ctx.time_sweeps(0, args.n_sweeps)
+    units = ctx.units_per_sweep()
+    print(json.dumps({"workload": p.name, "tuning": ctx.get_tuning(), "sweeps": args.n_sweeps,
+                      "ms_per_sweep": round(ms, 4), "launches_per_sweep": launches,
+                      "grind_ns": round(ms * 1e6 / units, 6)}), flush=True)
+    ctx.close()
+
+
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """A plain device-to-device copy of known size, for scaling HBM counters."""
+    import torch
+
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,11 +108,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
+    ap.add_argument("--mode", choices=["solve", "sweeps", "copy"], default="solve")
+    ap.add_argument("--workload", choices=["random_blocks", "multigroup_full"],
+                    default="random_blocks")
+    ap.add_argument("--shape", type=int, default=0, help="kernel shape id + 1 (sweeps mode)")
+    ap.add_argument("--schedule", type=int, default=0)
+    ap.add_argument("--n-sweeps", type=int, default=10)
+    ap.add_argument("--fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the sweep on a single MI355X "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "sweeps":
+        return sweeps_only(args)
+    if args.mode == "copy":
+        return copy_calibration()
 
     from doma_amd import workloads
     from doma_amd.capi import FP64, Context
@@ -112,11 +160,15 @@ def main():
     k_ms, w_ms, launches = ctx.profile_sweeps(0, 5)
     flops_sweep = units_sweep * FLOPS_PER_UNIT[p.scheme]
     achieved = flops_sweep / (k_ms * 1e-3) / 1e12
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
+    # valid only for the kernel shape and schedule that were profiled.
     traffic = None
     tfile = ROOT / "profiles" / "sweep_hbm_traffic.json"
     if tfile.exists():
-        tj = json.loads(tfile.read_text())
-        traffic = tj.get(f"{p.name}", {}).get("bytes_per_launch")
+        rec = json.loads(tfile.read_text()).get(p.name, {})
+        if (rec.get("tuning_shape") == tuning.get("shape")
+                and rec.get("schedule") == tuning.get("schedule")):
+            traffic = rec.get("bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                 "traffic": traffic,
