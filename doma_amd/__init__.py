@@ -7,10 +7,10 @@ every entry point raises when the library or a GPU is unavailable.
 """
 from .capi import (  # noqa: F401
     FP32_SWEEP, FP64, SCHEME_DD, SCHEME_TW_DD, Context, DomaError, MediaSpec, Problem,
-    gc_quadrature, gl_quadrature, lib, media_generate,
+    deck_read, gc_quadrature, gl_quadrature, lib, media_generate, run_deck,
 )
 
 __all__ = [
     "Context", "DomaError", "MediaSpec", "Problem", "gc_quadrature", "gl_quadrature", "lib",
-    "media_generate", "FP64", "FP32_SWEEP", "SCHEME_DD", "SCHEME_TW_DD",
+    "media_generate", "deck_read", "run_deck", "FP64", "FP32_SWEEP", "SCHEME_DD", "SCHEME_TW_DD",
 ]

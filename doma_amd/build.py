@@ -20,6 +20,7 @@ FRONTEND = PKG / "frontend"
 INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libdoma_amd.so"
+CLI = PKG / "doma"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DOMA_OFFLOAD_ARCH", "gfx950")
@@ -71,7 +72,23 @@ def build(force: bool = True, jobs: int | None = None) -> Path:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link of libdoma_amd.so failed")
     os.replace(tmp, LIB)
+    _link_cli(force)
     return LIB
+
+
+def _link_cli(force: bool) -> Path:
+    """Build the ``doma`` command-line driver against libdoma_amd.so ($ORIGIN rpath)."""
+    src = FRONTEND / "doma_main.cpp"
+    obj = _compile(src, force)
+    tmp = CLI.with_name("doma.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-o", str(tmp), str(obj), f"-L{PKG}", "-ldoma_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("link of the doma driver failed")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -73,6 +73,16 @@ class MediaSpec(C.Structure):
     ]
 
 
+class DeckInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "dims", "nx", "ny", "nz", "groups", "scheme", "mode", "np", "na", "num_materials", "dsa",
+        "refinement", "fs_max_it", "mg_max_it")] + [
+        ("fs_tol", C.c_double), ("mg_tol", C.c_double), ("total_volume", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _LIB = None
 
 # (name, restype, argtypes) of every exported entry point.
@@ -99,6 +109,10 @@ SIGNATURES = [
     ("doma_gl_quadrature", C.c_int, [C.c_int32, _dp, _dp]),
     ("doma_media_num_materials", C.c_int32, [C.POINTER(MediaSpec)]),
     ("doma_media_generate", C.c_int, [C.POINTER(MediaSpec), _ip, _dp, _dp, _dp, _u8p]),
+    ("doma_deck_read", C.c_int, [C.c_char_p, C.POINTER(DeckInfo)]),
+    ("doma_run_deck", C.c_int,
+     [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(Tuning), C.c_int32, _dp, C.c_int64,
+      C.POINTER(Stats), _ip]),
 ]
 
 
@@ -336,3 +350,34 @@ def media_generate(spec: MediaSpec):
     _check(lib().doma_media_generate(C.byref(spec), mat.ctypes.data_as(_ip), _d(sigt), _d(sigs),
                                      _d(src), has.ctypes.data_as(_u8p)), "doma_media_generate")
     return mat, sigt, sigs, src, has
+
+
+def deck_read(path) -> dict:
+    """Parse a <doma_inp> deck and expand its mesh (host only, no GPU needed)."""
+    info = DeckInfo()
+    _check(lib().doma_deck_read(str(path).encode(), C.byref(info)), "doma_deck_read")
+    return info.as_dict()
+
+
+def run_deck(path, output_base=None, precision: int = FP64, tuning: dict | None = None,
+             autotune: bool = False):
+    """Solve a fixed-source deck on the GPU like the reference's main():
+    returns (converged, phi[G, N], stats dict).  With output_base the
+    reference's text files <base>_*.txt are written."""
+    info = deck_read(path)
+    N = info["nx"] * info["ny"] * info["nz"]
+    phi = np.zeros((info["groups"], N))
+    st, conv = Stats(), C.c_int32()
+    t = None
+    if tuning:
+        t = Tuning(**tuning)
+    _check(lib().doma_run_deck(str(path).encode(),
+                               None if output_base is None else str(output_base).encode(),
+                               precision, None if t is None else C.byref(t), int(autotune),
+                               _d(phi), phi.size, C.byref(st), C.byref(conv)), "doma_run_deck")
+    G = info["groups"]
+    stats = {"success": st.success, "mg_iterations": st.mg_iterations,
+             "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+             "si_iterations": list(st.si_iterations[:G]), "si_residual": list(st.si_residual[:G]),
+             "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms}
+    return bool(conv.value), phi, stats

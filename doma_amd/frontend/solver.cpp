@@ -1,0 +1,300 @@
+// solver.cpp -- TransportSolver1D/2D/3D<T> over the C ABI.  Materials are the
+// deck's blocks (MaterialProps per block id), mapped to dense material indices;
+// quadratures are the reference's (GCAngularQuadrature(2na, 2np, 3|2),
+// GLAngularQuadrature(2np): TransportSolver3D.cpp:10, TransportSolver2D.cpp:10,
+// TransportSolver1D.cpp:12).  Console lines mirror the reference's summaries.
+#include "solver.h"
+
+#include <iostream>
+#include <map>
+
+namespace doma
+{
+SolverCore::SolverCore(BrickMesh & mesh, const InputParameters & params, bool verbose, int scheme)
+  : _mesh(mesh), _params(params), _verbose(verbose), _scheme(scheme)
+{
+  _mesh.setNumGroups(params._num_e_groups);
+}
+
+bool
+SolverCore::refuse(const std::string & why)
+{
+  _error = why;
+  std::cout << why << std::endl;
+  return false;
+}
+
+bool
+SolverCore::fixedSourceOnly()
+{
+  _error.clear();
+  if (_params._mode == RunMode::FixedSrc)
+    return true;
+  return refuse("This build solves fixed-source problems only (solve_type=\"steady\").");
+}
+
+bool
+SolverCore::run(const std::string & output_file_base, int dims, int dirs,
+                const std::vector<double> & mu, const std::vector<double> & eta,
+                const std::vector<double> & xi, const std::vector<double> & w)
+{
+  _error.clear();
+  const unsigned int G = _params._num_e_groups;
+  if (G < 1 || G > DOMA_MAX_GROUPS)
+  {
+    _error = "num_groups must be in [1, 64]";
+    std::cerr << _error << std::endl;
+    return false;
+  }
+  // Dense material indices for the block ids in use, ascending by block id.
+  std::map<unsigned int, int> index;
+  for (const auto & kv : _params._block_mat_info)
+    index.emplace(kv.first, 0);
+  int m = 0;
+  for (auto & kv : index)
+    kv.second = m++;
+  const int M = m;
+  std::vector<double> sigt((size_t)M * G), sigs((size_t)M * G * G), src((size_t)M * G, 0.0);
+  std::vector<uint8_t> has(M, 0);
+  for (const auto & kv : index)
+  {
+    const MaterialProps & p = _params._block_mat_info.at(kv.first);
+    const int mi = kv.second;
+    for (unsigned int g = 0; g < G; ++g)
+      sigt[(size_t)mi * G + g] = p._g_total[g];
+    for (unsigned int e = 0; e < G * G; ++e)
+      sigs[(size_t)mi * G * G + e] = p._g_g_scatter_mat[e];
+    if (!p._g_src.empty())
+    {
+      has[mi] = 1;
+      for (unsigned int g = 0; g < G; ++g)
+        src[(size_t)mi * G + g] = p._g_src[g];
+    }
+  }
+  const unsigned int N = _mesh.numCells();
+  std::vector<int32_t> mat(N);
+  for (unsigned int c = 0; c < N; ++c)
+  {
+    auto it = index.find(_mesh.blockIds()[c]);
+    if (it == index.end())
+    {
+      _error = "Block " + std::to_string(_mesh.blockIds()[c]) +
+               " (defined on the mesh) has no material properties!";
+      std::cerr << _error << std::endl;
+      return false;
+    }
+    mat[c] = it->second;
+  }
+
+  doma_problem_desc d{};
+  d.dims = dims;
+  d.nx = (int)_mesh.numX();
+  d.ny = (int)_mesh.numY();
+  d.nz = (int)_mesh.numZ();
+  d.lx = _mesh.widthsX().data();
+  d.ly = _mesh.widthsY().data();
+  d.lz = _mesh.widthsZ().data();
+  d.groups = (int)G;
+  d.scheme = _scheme;
+  d.precision = _opt.precision;
+  d.dirs = dirs;
+  d.mu = mu.data();
+  d.eta = eta.data();
+  d.xi = xi.data();
+  d.w = w.data();
+
+  std::cout << "Solving..." << std::endl;
+  std::cout << "Initializing the solver..." << std::endl;
+  doma_ctx * ctx = nullptr;
+  int rc = doma_create(&d, &ctx);
+  if (rc == DOMA_OK)
+    rc = doma_set_materials(ctx, mat.data(), M, sigt.data(), sigs.data(), src.data(), has.data());
+  if (rc == DOMA_OK && (_opt.tuning.brick || _opt.tuning.threads || _opt.tuning.exact_arith ||
+                        _opt.tuning.schedule || _opt.tuning.octants || _opt.tuning.dirs_per_lane ||
+                        _opt.tuning.shape))
+    rc = doma_set_tuning(ctx, &_opt.tuning);
+  if (rc == DOMA_OK && _opt.autotune)
+    rc = doma_autotune(ctx, 0, nullptr, nullptr);
+  if (rc == DOMA_OK)
+    rc = doma_solve_fixed_source(ctx, _params._src_it_tol, (int)_params._num_src_it, _params._gs_tol,
+                                 (int)_params._num_mg_it, _mesh.flux().data(), &_stats);
+  if (rc != DOMA_OK)
+  {
+    _error = doma_last_error();
+    std::cerr << "doma: " << _error << std::endl;
+    doma_destroy(ctx);
+    return false;
+  }
+  doma_destroy(ctx);
+
+  for (unsigned int g = 0; g < G && g < DOMA_MAX_GROUPS; ++g)
+  {
+    if (_stats.si_iterations[g] == 0)
+      break;
+    const bool conv = _stats.si_iterations[g] < (int)_params._num_src_it;
+    std::cout << (conv ? "SI converged after " : "SI failed to converge after ")
+              << _stats.si_iterations[g] << " iterations with a residual of "
+              << _stats.si_residual[g] << " for G" << g << std::endl;
+  }
+  if (_stats.success)
+  {
+    if (G > 1)
+      std::cout << "MGI converged after " << _stats.mg_iterations << " iterations with a residual of "
+                << _stats.mg_residual << std::endl;
+    if (!output_file_base.empty())
+      _mesh.dumpToTextFile(output_file_base);
+  }
+  else if (_stats.mg_iterations >= (int)_params._num_mg_it)
+    std::cout << "MGI failed to converge after " << _stats.mg_iterations
+              << " iterations with a residual of " << _stats.mg_residual << std::endl;
+  if (_verbose)
+    std::cout << "GPU: " << _stats.sweeps << " group sweeps, " << _stats.sweep_ms
+              << " ms in sweep kernels, " << _stats.elapsed_ms << " ms total" << std::endl;
+  return _stats.success != 0;
+}
+
+namespace
+{
+// Flatten a product quadrature into [octant slot][n] arrays; 2D keeps the four
+// slots with xi > 0 (PPP, PMP, MPP, MMP).
+int
+flatten(const GCAngularQuadrature & q, int dims, std::vector<double> & mu, std::vector<double> & eta,
+        std::vector<double> & xi, std::vector<double> & w)
+{
+  const int slots = dims == 3 ? 8 : 4;
+  const int per = (int)q.order(Octant::PPP);
+  for (int s = 0; s < slots; ++s)
+  {
+    const auto oct = static_cast<Octant>(dims == 3 ? s : 2 * s);
+    for (int n = 0; n < per; ++n)
+    {
+      double a, b, c;
+      q.direction(oct, n, a, b, c);
+      mu.push_back(a);
+      eta.push_back(b);
+      xi.push_back(c);
+      w.push_back(q.weight(oct, n));
+    }
+  }
+  return per;
+}
+} // namespace
+
+// ---------------------------------------------------------------- 3D ------- //
+
+template <typename T>
+TransportSolver3D<T>::TransportSolver3D(BrickMesh3D & mesh, const InputParameters & params,
+                                        bool verbose, unsigned int)
+  : SolverCore(mesh, params, verbose, T::scheme),
+    _angular_quad(2u * params._num_azimuthal, 2u * params._num_polar, 3u)
+{
+}
+
+template <typename T>
+bool
+TransportSolver3D<T>::solveFixedSource(const std::string & output_file_base, const double &)
+{
+  if (!fixedSourceOnly())
+    return false;
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 3, mu, eta, xi, w);
+  return run(output_file_base, 3, per, mu, eta, xi, w);
+}
+
+template <typename T>
+bool
+TransportSolver3D<T>::solveEigenvalue(const std::string &)
+{
+  return refuse("Eigenvalue mode is outside this build's scope.");
+}
+
+template <typename T>
+bool
+TransportSolver3D<T>::solveTransient(const std::string &)
+{
+  return refuse("Transient mode is outside this build's scope.");
+}
+
+// ---------------------------------------------------------------- 2D ------- //
+
+template <typename T>
+TransportSolver2D<T>::TransportSolver2D(BrickMesh2D & mesh, const InputParameters & params,
+                                        bool verbose, unsigned int)
+  : SolverCore(mesh, params, verbose, T::scheme),
+    _angular_quad(2u * params._num_azimuthal, 2u * params._num_polar, 2u)
+{
+}
+
+template <typename T>
+bool
+TransportSolver2D<T>::solveFixedSource(const std::string & output_file_base, const double &)
+{
+  if (!fixedSourceOnly())
+    return false;
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 2, mu, eta, xi, w);
+  return run(output_file_base, 2, per, mu, eta, xi, w);
+}
+
+template <typename T>
+bool
+TransportSolver2D<T>::solveEigenvalue(const std::string &)
+{
+  return refuse("Eigenvalue mode is outside this build's scope.");
+}
+
+template <typename T>
+bool
+TransportSolver2D<T>::solveTransient(const std::string &)
+{
+  return refuse("Transient mode is outside this build's scope.");
+}
+
+// ---------------------------------------------------------------- 1D ------- //
+
+template <typename T>
+TransportSolver1D<T>::TransportSolver1D(BrickMesh1D & mesh, const InputParameters & params,
+                                        bool verbose, bool dsa, unsigned int)
+  : SolverCore(mesh, params, verbose, T::scheme), _angular_quad(2u * params._num_polar), _dsa(dsa)
+{
+}
+
+template <typename T>
+bool
+TransportSolver1D<T>::solveFixedSource(const std::string & output_file_base, const double &)
+{
+  if (!fixedSourceOnly())
+    return false;
+  if (_dsa)
+    return refuse("Diffusion synthetic acceleration is outside this build's scope; solve with "
+                  "dsa=\"false\".");
+  const int n = (int)_angular_quad.order();
+  std::vector<double> mu(n), eta(n, 0.0), xi(n, 0.0), w(n);
+  for (int i = 0; i < n; ++i)
+  {
+    mu[i] = _angular_quad.direction(i);
+    w[i] = _angular_quad.weight(i);
+  }
+  return run(output_file_base, 1, n / 2, mu, eta, xi, w);
+}
+
+template <typename T>
+bool
+TransportSolver1D<T>::solveEigenvalue(const std::string &)
+{
+  return refuse("Eigenvalue mode is outside this build's scope.");
+}
+
+template <typename T>
+bool
+TransportSolver1D<T>::solveTransient(const std::string &)
+{
+  return refuse("Transient mode is outside this build's scope.");
+}
+
+template class TransportSolver3D<TWDiamondDifference3D>;
+template class TransportSolver3D<DiamondDifference3D>;
+template class TransportSolver2D<TWDiamondDifference2D>;
+template class TransportSolver2D<DiamondDifference2D>;
+template class TransportSolver1D<DiamondDifference1D>;
+} // namespace doma

@@ -151,6 +151,35 @@ int64_t doma_units_per_sweep(const doma_ctx *ctx);
 int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
 
 /* ------------------------------------------------------------------------- */
+/* Input decks (host C++ front end).                                           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct doma_deck_info {
+  int32_t dims, nx, ny, nz;      /* cells per axis after interval expansion and refinement */
+  int32_t groups, scheme, mode;  /* mode: 0 steady (fixed source), 1 eigen, 2 transient */
+  int32_t np, na;                /* polar / azimuthal quadrature parameters */
+  int32_t num_materials, dsa, refinement;
+  int32_t fs_max_it, mg_max_it;
+  double fs_tol, mg_tol;
+  double total_volume;
+} doma_deck_info;
+
+/* Read a <doma_inp> deck (src/InputParameters.cpp:91-524) and expand its mesh
+ * (src/BrickMesh3D.cpp:8-197).  Host only; invalid decks give DOMA_ERR_INVALID
+ * with the reference's message in doma_last_error(). */
+int doma_deck_read(const char *path, doma_deck_info *out);
+
+/* Run a fixed-source deck end to end like the reference's main()
+ * (src/main.cpp:51-298): mesh, solver by dimension and scheme, GPU solve; with a
+ * non-NULL output_base the reference's text files <base>_dims.txt, _blocks.txt,
+ * _meshx/y/z.txt, _g<g>_flux.txt are written.  phi (may be NULL) receives [G][N]
+ * of capacity phi_len doubles; *converged is the reference's solveFixedSource
+ * result.  tuning may be NULL; autotune != 0 runs doma_autotune first. */
+int doma_run_deck(const char *path, const char *output_base, int32_t precision,
+                  const doma_tuning *tuning, int32_t autotune, double *phi, int64_t phi_len,
+                  doma_stats *stats, int32_t *converged);
+
+/* ------------------------------------------------------------------------- */
 /* Angular quadratures (host only).                                            */
 /* ------------------------------------------------------------------------- */
 

@@ -1,0 +1,145 @@
+"""Writes the deck fixtures under tests/golden/decks/ by running the reference's
+own compiled mesh and solver units (oracle/_ref/ref_deck, built from
+/root/reference by ``make -C oracle ref``) on the reference's input decks.
+Run in the build container only:
+
+    python tests/golden/make_deck_golden.py [deck ...]
+
+The decks are read here with Python's xml.etree (an XML reader independent of
+the product's own) into the InputParameters fields the reference's parser
+produces (src/InputParameters.cpp:91-524: ix *= refinement + 1, scatter
+matrices dest-major, transposed when transpose="true").  Per deck the script
+stores
+
+  decks/<deck>.xml  the deck's elements re-serialised (comments dropped), as
+                    input data for the GPU tests;
+  decks/<deck>.npz  the reference's scalar flux (all of it, or a seeded sample
+                    with per-group sums for large meshes), its success flag and
+                    SI/MG iteration counts, and the SHA-256 and line count of
+                    every text file the reference wrote (<base>_dims.txt,
+                    _blocks.txt, _meshx/y/z.txt, _g<g>_flux.txt).
+
+For 1D decks only the mesh is built (the reference's 1D solver needs Eigen, which
+is absent): the mesh files are pinned, the flux is left to the oracle.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import subprocess
+import sys
+import tempfile
+import xml.etree.ElementTree as ET
+from pathlib import Path
+
+import numpy as np
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+from deck_xml import read_deck  # noqa: E402
+
+ROOT = Path(__file__).resolve().parents[2]
+REF = Path("/root/reference")
+OUT = Path(__file__).resolve().parent / "decks"
+HARNESS = ROOT / "oracle" / "_ref" / "ref_deck"
+
+# deck name -> path under the reference tree; all fixed-source decks it holds
+# that the reference's 2D/3D solver (or 1D mesh) runs in a few minutes here.
+DECKS = {
+    "sample_params": "sample_params.xml",
+    "2D_watanabe": "cases/2D_watanabe.xml",
+    "2D_test": "cases/2D_test.xml",
+    "2D_assembly_fs": "cases/2D_assembly_fs.xml",
+    "3D_kobayashi_1": "cases/3D_kobayashi_1.xml",
+    "3D_kobayashi_3": "cases/3D_kobayashi_3.xml",
+    "1D_reed": "cases/1D_reed.xml",
+    "1D_NPRE_555_mp2": "cases/1D_NPRE_555_mp2.xml",
+    "1D_NPRE_555_mp3_3g_src": "cases/1D_NPRE_555_mp3_3g_src.xml",
+}
+FULL_LIMIT = 40000  # store the whole flux up to this many values
+THREADS = 8
+
+
+def write_spec(p, path, run):
+    lines = [f"{p['dims']} {p['G']} {p['scheme']} {p['np']} {p['na']} {p['fs_max_it']} "
+             f"{p['mg_max_it']} {THREADS} {run}", f"{p['fs_tol']!r} {p['mg_tol']!r}"]
+    for ax in "xyz"[:p["dims"]]:
+        d, n = p["d" + ax], p["i" + ax]
+        lines.append(f"{len(d)} " + " ".join(repr(v) for v in d) + " " + " ".join(map(str, n)))
+    lines.append(f"{len(p['blocks'])} " + " ".join(map(str, p["blocks"])))
+    lines.append(str(len(p["materials"])))
+    for bid, d in sorted(p["materials"].items()):
+        has = d["src"] is not None
+        vals = d["sigt"] + d["sigs"] + (d["src"] if has else [])
+        lines.append(f"{bid} {int(has)} " + " ".join(repr(v) for v in vals))
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def _files(base, dims, G):
+    names = ["dims", "blocks", "meshx"] + (["meshy"] if dims > 1 else []) + \
+        (["meshz"] if dims > 2 else []) + [f"g{g}_flux" for g in range(G)]
+    return {n: Path(f"{base}_{n}.txt") for n in names}
+
+
+def make(name, tmp):
+    src = REF / DECKS[name]
+    root, p = read_deck(src)
+    assert p["mode"] == "steady", name
+    OUT.mkdir(exist_ok=True)
+    ET.ElementTree(root).write(OUT / f"{name}.xml", encoding="unicode")
+    run = 1 if p["dims"] > 1 else 0
+    spec, base = Path(tmp) / f"{name}.spec", str(Path(tmp) / name)
+    write_spec(p, spec, run)
+    subprocess.run([str(HARNESS), str(spec), base], check=True)
+    raw = Path(base + ".bin").read_bytes()
+    ok = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    phi = np.frombuffer(raw[4:], dtype=np.float64)
+    files = _files(base, p["dims"], p["G"])
+    if not run:
+        files = {k: v for k, v in files.items() if not k.endswith("_flux")}
+    arrays = dict(ref_success=ok, dims=p["dims"], groups=p["G"])
+    meta = {"success": ok, "files": {}}
+    for k, f in files.items():
+        data = f.read_bytes()
+        arrays[f"sha256_{k}"] = hashlib.sha256(data).hexdigest()
+        arrays[f"lines_{k}"] = data.count(b"\n")
+        meta["files"][k] = arrays[f"lines_{k}"]
+    if run:
+        N = phi.size // p["G"]
+        phi = phi.reshape(p["G"], N)
+        # The reference writes each value with operator<< at precision 6, which
+        # is printf's %.6g: check that equivalence once here, on its own files.
+        for g in range(p["G"]):
+            text = files[f"g{g}_flux"].read_text().splitlines()
+            assert text == ["%.6g" % v for v in phi[g]], (name, g)
+        if phi.size <= FULL_LIMIT:
+            arrays["phi"] = phi
+        else:
+            rng = np.random.default_rng(7)
+            idx = np.sort(rng.choice(N, size=4096, replace=False)).astype(np.int64)
+            arrays["sample_idx"] = idx
+            arrays["phi_sample"] = phi[:, idx]
+            arrays["phi_group_l2"] = np.sqrt((phi ** 2).sum(axis=1))
+        log = Path(base + ".log").read_text()
+        sys.path.insert(0, str(Path(__file__).resolve().parent))
+        from make_golden import parse_log  # noqa: E402
+        meta["log"] = parse_log(log)
+        arrays["si_iterations"] = np.array([a for a, _, _ in meta["log"]["si"]], dtype=np.int64)
+    np.savez_compressed(OUT / f"{name}.npz", **arrays)
+    return meta
+
+
+def main(argv):
+    names = argv or list(DECKS)
+    OUT.mkdir(exist_ok=True)
+    man_path = OUT / "manifest.json"
+    manifest = json.loads(man_path.read_text()) if man_path.exists() else {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for n in names:
+            print("deck", n, flush=True)
+            manifest[n] = make(n, tmp)
+            print("  ", manifest[n]["success"], flush=True)
+    man_path.write_text(json.dumps(manifest, indent=1, sort_keys=True))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
