@@ -6,11 +6,11 @@ The compute path is the native library ``libdoma_amd.so`` (C ABI in
 every entry point raises when the library or a GPU is unavailable.
 """
 from .capi import (  # noqa: F401
-    FP32_SWEEP, FP64, SCHEME_DD, SCHEME_TW_DD, Context, DomaError, MediaSpec, Problem,
+    FP32_SWEEP, FP64, SCHEME_DD, SCHEME_TW_DD, Batch, Context, DomaError, MediaSpec, Problem,
     deck_read, gc_quadrature, gl_quadrature, lib, media_generate, run_deck,
 )
 
 __all__ = [
-    "Context", "DomaError", "MediaSpec", "Problem", "gc_quadrature", "gl_quadrature", "lib",
+    "Batch", "Context", "DomaError", "MediaSpec", "Problem", "gc_quadrature", "gl_quadrature", "lib",
     "media_generate", "deck_read", "run_deck", "FP64", "FP32_SWEEP", "SCHEME_DD", "SCHEME_TW_DD",
 ]

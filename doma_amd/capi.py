@@ -83,6 +83,14 @@ class DeckInfo(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("rounds", C.c_int32), ("units", C.c_int64), ("elapsed_ms", C.c_double),
+                ("sweep_ms", C.c_double), ("samples", C.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _LIB = None
 
 # (name, restype, argtypes) of every exported entry point.
@@ -109,6 +117,20 @@ SIGNATURES = [
     ("doma_gl_quadrature", C.c_int, [C.c_int32, _dp, _dp]),
     ("doma_media_num_materials", C.c_int32, [C.POINTER(MediaSpec)]),
     ("doma_media_generate", C.c_int, [C.POINTER(MediaSpec), _ip, _dp, _dp, _dp, _u8p]),
+    ("doma_batch_create", C.c_int, [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
+    ("doma_batch_destroy", C.c_int, [C.c_void_p]),
+    ("doma_batch_set_materials", C.c_int, [C.c_void_p, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp,
+                                           _u8p]),
+    ("doma_batch_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("doma_batch_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    ("doma_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(Tuning), _dp]),
+    ("doma_batch_time_sweeps", C.c_int, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    ("doma_batch_solve", C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32,
+                                   C.c_int32, C.c_int32, C.POINTER(Stats), C.POINTER(BatchInfo)]),
+    ("doma_batch_get_flux", C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    ("doma_batch_sample_count", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _ip]),
+    ("doma_batch_get_sample", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("doma_deck_read", C.c_int, [C.c_char_p, C.POINTER(DeckInfo)]),
     ("doma_run_deck", C.c_int,
      [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(Tuning), C.c_int32, _dp, C.c_int64,
@@ -336,6 +358,103 @@ class Context:
         t, ms = Tuning(), C.c_double()
         _check(lib().doma_autotune(self.h, g, C.byref(t), C.byref(ms)), "doma_autotune")
         return t.as_dict(), ms.value
+
+
+def _stats_dict(st: Stats, G: int) -> dict:
+    return {"success": bool(st.success), "mg_iterations": st.mg_iterations,
+            "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+            "si_iterations": list(st.si_iterations[:G]), "si_residual": list(st.si_residual[:G]),
+            "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms}
+
+
+class Batch:
+    """Owns a doma_batch: many independent 3D problems (same scheme, precision and
+    quadrature) swept together, one persistent launch per source-iteration round."""
+
+    def __init__(self, problems, precision: int = FP64):
+        ps = [p.normalized() for p in problems]
+        self.problems = ps
+        descs = (ProblemDesc * len(ps))()
+        for i, p in enumerate(ps):
+            descs[i] = ProblemDesc(p.dims, p.nx, p.ny, p.nz, _d(p.lx), _d(p.ly), _d(p.lz),
+                                   p.groups, p.scheme, precision, p.dirs, _d(p.mu), _d(p.eta),
+                                   _d(p.xi), _d(p.w))
+        h = C.c_void_p()
+        _check(lib().doma_batch_create(len(ps), descs, C.byref(h)), "doma_batch_create")
+        self.h = h
+        for i, p in enumerate(ps):
+            _check(lib().doma_batch_set_materials(h, i, p.mat.ctypes.data_as(_ip), p.num_mats,
+                                                  _d(p.sigt), _d(p.sigs), _d(p.src),
+                                                  p.has_src.ctypes.data_as(_u8p)),
+                   "doma_batch_set_materials")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().doma_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, **kw):
+        t = Tuning(**kw)
+        _check(lib().doma_batch_set_tuning(self.h, C.byref(t)), "doma_batch_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().doma_batch_get_tuning(self.h, C.byref(t)), "doma_batch_get_tuning")
+        return t.as_dict()
+
+    def autotune(self):
+        t, ms = Tuning(), C.c_double()
+        _check(lib().doma_batch_autotune(self.h, C.byref(t), C.byref(ms)), "doma_batch_autotune")
+        return t.as_dict(), ms.value
+
+    def time_sweeps(self, n: int):
+        ms, units = C.c_double(), C.c_int64()
+        _check(lib().doma_batch_time_sweeps(self.h, n, C.byref(ms), C.byref(units)),
+               "doma_batch_time_sweeps")
+        return ms.value, units.value
+
+    def solve(self, fs_tol: float, fs_max_it: int, mg_tol: float, mg_max_it: int,
+              sample_every: int = 0, sample_cap: int = 0):
+        st = (Stats * len(self.problems))()
+        info = BatchInfo()
+        _check(lib().doma_batch_solve(self.h, fs_tol, fs_max_it, mg_tol, mg_max_it, sample_every,
+                                      sample_cap, st, C.byref(info)), "doma_batch_solve")
+        return [_stats_dict(st[i], p.groups) for i, p in enumerate(self.problems)], info.as_dict()
+
+    def get_flux(self, p: int) -> np.ndarray:
+        pr = self.problems[p]
+        phi = np.zeros(pr.groups * pr.N)
+        _check(lib().doma_batch_get_flux(self.h, p, _d(phi)), "doma_batch_get_flux")
+        return phi.reshape(pr.groups, pr.N)
+
+    def sample_iterations(self, p: int, g: int):
+        n = C.c_int32()
+        _check(lib().doma_batch_sample_count(self.h, p, g, C.byref(n), None),
+               "doma_batch_sample_count")
+        its = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(lib().doma_batch_sample_count(self.h, p, g, C.byref(n), its.ctypes.data_as(_ip)),
+               "doma_batch_sample_count")
+        return its[: n.value].tolist()
+
+    def get_sample(self, p: int, g: int, k: int):
+        N = self.problems[p].N
+        r, t = np.zeros(N, dtype=np.float32), np.zeros(N, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(lib().doma_batch_get_sample(self.h, p, g, k, r.ctypes.data_as(fp),
+                                           t.ctypes.data_as(fp)), "doma_batch_get_sample")
+        return r, t
 
 
 def media_generate(spec: MediaSpec):

@@ -322,23 +322,10 @@ configure_sweep(doma_ctx * c)
   return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
 }
 
-// One sweep of group g over every octant: q = c->d_q, sigma_t = c->d_sig,
-// partials -> c->d_part.
-int
-run_sweep_with(doma_ctx * c, const double * q)
+// Sweep arguments of a 2D/3D context for source q (schedule fields left zero).
+BrickArgs
+brick_args(doma_ctx * c, const double * q)
 {
-  if (c->dims == 1)
-  {
-    Sweep1DArgs a{};
-    a.nx = c->nx;
-    a.dirs = c->dirs;
-    a.dirtab = c->d_dirtab;
-    a.lx = c->d_lx;
-    a.q = q;
-    a.sig = c->d_sig;
-    a.part = c->d_part;
-    return launch_sweep1d(a, c->precision, c->stream);
-  }
   BrickArgs a{};
   a.nx = c->nx;
   a.ny = c->ny;
@@ -370,6 +357,27 @@ run_sweep_with(doma_ctx * c, const double * q)
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.ntz = c->ntz;
+  return a;
+}
+
+// One sweep of group g over every octant: q = c->d_q, sigma_t = c->d_sig,
+// partials -> c->d_part.
+int
+run_sweep_with(doma_ctx * c, const double * q)
+{
+  if (c->dims == 1)
+  {
+    Sweep1DArgs a{};
+    a.nx = c->nx;
+    a.dirs = c->dirs;
+    a.dirtab = c->d_dirtab;
+    a.lx = c->d_lx;
+    a.q = q;
+    a.sig = c->d_sig;
+    a.part = c->d_part;
+    return launch_sweep1d(a, c->precision, c->stream);
+  }
+  BrickArgs a = brick_args(c, q);
   if (c->schedule == 1)
   {
     a.ntasks = c->ntasks;
@@ -495,8 +503,12 @@ doma_version(void)
   return "doma_amd 0.1 (gfx950)";
 }
 
-extern "C" int
-doma_create(const doma_problem_desc * d, doma_ctx ** out)
+namespace doma
+{
+// Context creation; with a non-null shared stream the context issues its work on
+// that stream (batches) and does not own it.
+int
+create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
 {
   if (!d || !out)
     return fail(DOMA_ERR_INVALID, "doma_create: null argument");
@@ -571,8 +583,14 @@ doma_create(const doma_problem_desc * d, doma_ctx ** out)
         khz > 0)
       c->wall_clock_khz = khz;
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+  if (shared)
+  {
+    c->stream = shared;
+    c->own_stream = false;
+  }
+  else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(DOMA_ERR_DEVICE, "stream creation failed"));
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "stream/event creation failed"));
 
   std::vector<double> lyv(ny, 1.0), lzv(nz, 1.0);
@@ -649,6 +667,13 @@ doma_create(const doma_problem_desc * d, doma_ctx ** out)
   *out = c;
   return DOMA_OK;
 }
+} // namespace doma
+
+extern "C" int
+doma_create(const doma_problem_desc * d, doma_ctx ** out)
+{
+  return create_ctx(d, nullptr, out);
+}
 
 extern "C" int
 doma_destroy(doma_ctx * c)
@@ -688,7 +713,7 @@ doma_destroy(doma_ctx * c)
     (void)hipEventDestroy(c->ev1);
   for (auto e : c->prof_events)
     (void)hipEventDestroy(e);
-  if (c->stream)
+  if (c->stream && c->own_stream)
     (void)hipStreamDestroy(c->stream);
   delete c;
   return DOMA_OK;

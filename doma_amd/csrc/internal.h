@@ -63,6 +63,25 @@ struct BrickArgs
   long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
 };
 
+// Arguments of the batched persistent sweep: many 3D problems swept with one
+// kernel shape in one launch, the tasks of all of them in one ticket order.  A
+// task code is (problem << 20) | (brick << 6) | pair, where brick indexes the
+// problem's diagonal-sorted brick list and pair = octant group * nchunks + chunk.
+struct BatchArgs
+{
+  const BrickArgs * probs;         // [P] per-problem sweep arguments (device)
+  const uint32_t * tasks;          // [ntasks]
+  int ntasks;
+  uint32_t epoch;                  // > every flag value of every problem before it
+  unsigned long long * ticket;     // monotone ticket counter (never reset)
+  unsigned long long ticket_base;  // counter value at the start of this launch
+  int * error;                     // set when a bounded wait expires
+  long long wait_ticks;            // per-dependency deadline in wall-clock ticks
+};
+constexpr int kBatchMaxProblems = 1 << 12;
+constexpr int kBatchMaxBricks = 1 << 14;
+constexpr int kBatchMaxPairs = 1 << 6;
+
 struct Sweep1DArgs
 {
   int nx;
@@ -90,6 +109,10 @@ int launch_brick_sweep(const BrickArgs & a, int scheme, int precision, int exact
 int sweep_blocks_per_cu(int scheme, int precision, int exact, int shape, int schedule,
                         int * per_cu);
 int launch_sweep1d(const Sweep1DArgs & a, int precision, hipStream_t s);
+// Batched persistent sweep (3D shapes only, fast arithmetic).
+int launch_brick_batch(const BatchArgs & b, int scheme, int precision, int shape, int grid,
+                       hipStream_t s);
+int batch_blocks_per_cu(int scheme, int precision, int shape, int * per_cu);
 
 // Launch wrappers implemented in iterate.hip.
 struct IterArgs
@@ -127,6 +150,14 @@ int launch_residual_final(const IterArgs & a, int nblocks, hipStream_t s);
 int launch_reduce_parts(const IterArgs & a, hipStream_t s);
 int launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s);
 int launch_fill(double * p, size_t n, double v, hipStream_t s);
+// Batched finish/residual over problem slots (see iterate.hip), training-data
+// snapshots and targets.
+int launch_batch_finish(const IterArgs * d_args, const int4 * d_map, int nblocks, hipStream_t s);
+int launch_batch_residual(const IterArgs * d_args, const int * d_nblocks, int nprob,
+                          hipStream_t s);
+int launch_capture(const IterArgs & a, float * res_out, double * phi_out, hipStream_t s);
+int launch_targets(const double * phi_g, const double * snaps, float * targets, int N, int count,
+                   hipStream_t s);
 int launch_set_q(const IterArgs & a, const double * qmg_dev, hipStream_t s);
 int reduction_blocks(int N);
 } // namespace doma
@@ -138,6 +169,7 @@ struct doma_ctx
   int N = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   // geometry / quadrature (device)
@@ -207,9 +239,11 @@ struct doma_ctx
 
 namespace doma
 {
+int create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out);
 int select_shape(int dims, int noct, int chunk, int minext, const doma_tuning & t);
 int configure_sweep(doma_ctx * c);
 int run_sweep(doma_ctx * c, int g);
+BrickArgs brick_args(doma_ctx * c, const double * q);
 int check_sweep_error(doma_ctx * c);
 int reset_after_sweep_error(doma_ctx * c);
 void free_sweep(doma_ctx * c);

@@ -105,14 +105,15 @@ mg_source_kernel(IterArgs a)
 
 // Sum of the direction-set partials, the scattering residual terms and the
 // scattering-source update, fused.  2D/3D increment form: TransportSolver3D.cpp:
-// 531-563; 1D standard form: TransportSolver1D.cpp:802-838.
-__global__ __launch_bounds__(RED_THREADS) void
-finish_kernel(IterArgs a)
+// 531-563; 1D standard form: TransportSolver1D.cpp:802-838.  Block `block` of
+// `nblocks` walks a fixed strided set of cells.
+__device__ __forceinline__ void
+finish_body(const IterArgs & a, int block, int nblocks)
 {
 #pragma clang fp contract(off)
   double diff = 0.0, total = 0.0;
-  const int stride = gridDim.x * RED_THREADS;
-  for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < a.N; c += stride)
+  const int stride = nblocks * RED_THREADS;
+  for (int c = block * RED_THREADS + threadIdx.x; c < a.N; c += stride)
   {
     double d = 0.0;
     for (int s = 0; s < a.ndirsets; ++s)
@@ -142,15 +143,21 @@ finish_kernel(IterArgs a)
   block_sum2(diff, total);
   if (threadIdx.x == 0)
   {
-    a.red[2 * blockIdx.x + 0] = diff;
-    a.red[2 * blockIdx.x + 1] = total;
+    a.red[2 * block + 0] = diff;
+    a.red[2 * block + 1] = total;
   }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void
+finish_kernel(IterArgs a)
+{
+  finish_body(a, blockIdx.x, gridDim.x);
 }
 
 // Final fixed-order sum of block partials; res[0] = residual as the reference
 // forms it (0 when the squared norm is <= 1e-8, TransportSolver3D.cpp:562).
-__global__ __launch_bounds__(RED_THREADS) void
-residual_final_kernel(IterArgs a, int nblocks)
+__device__ __forceinline__ void
+residual_body(const IterArgs & a, int nblocks)
 {
   double diff = 0.0, total = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += RED_THREADS)
@@ -166,6 +173,57 @@ residual_final_kernel(IterArgs a, int nblocks)
     a.res[2] = total;
     a.res[3] = a.err ? static_cast<double>(*a.err) : 0.0;
   }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void
+residual_final_kernel(IterArgs a, int nblocks)
+{
+  residual_body(a, nblocks);
+}
+
+// Batched finish: block b works for problem slot map[b].x as its block map[b].y
+// of map[b].z; the problems' own block counts keep their reductions identical to
+// the unbatched ones.
+__global__ __launch_bounds__(RED_THREADS) void
+batch_finish_kernel(const IterArgs * args, const int4 * map)
+{
+  const int4 m = map[blockIdx.x];
+  const IterArgs a = args[m.x];
+  finish_body(a, m.y, m.z);
+}
+
+// Batched residual: block b reduces problem slot b.
+__global__ __launch_bounds__(RED_THREADS) void
+batch_residual_kernel(const IterArgs * args, const int * nblocks)
+{
+  const IterArgs a = args[blockIdx.x];
+  residual_body(a, nblocks[blockIdx.x]);
+}
+
+// Training-data snapshot after a source iteration: the latest increment (the SI
+// residual field, FP32) and the accumulated scalar flux of the group (FP64).
+__global__ void
+capture_kernel(IterArgs a, float * res_out, double * phi_out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  double d = 0.0;
+  for (int s = 0; s < a.ndirsets; ++s)
+    d += a.part[(size_t)s * a.N + c];
+  res_out[c] = static_cast<float>(d);
+  phi_out[c] = a.phi[(size_t)a.g * a.N + c];
+}
+
+// Targets of the stored snapshots once the group's iteration has converged:
+// target_k = phi_converged - phi_k, in FP32.
+__global__ void
+target_kernel(const double * phi_g, const double * snaps, float * targets, int N, int count)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * count)
+    return;
+  targets[i] = static_cast<float>(phi_g[i % (size_t)N] - snaps[i]);
 }
 
 __global__ void
@@ -321,6 +379,41 @@ launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s)
 {
   hipLaunchKernelGGL(mg_residual_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_batch_finish(const IterArgs * d_args, const int4 * d_map, int nblocks, hipStream_t s)
+{
+  if (nblocks > 0)
+    hipLaunchKernelGGL(batch_finish_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, d_args, d_map);
+  return DOMA_OK;
+}
+
+int
+launch_batch_residual(const IterArgs * d_args, const int * d_nblocks, int nprob, hipStream_t s)
+{
+  if (nprob > 0)
+    hipLaunchKernelGGL(batch_residual_kernel, dim3(nprob), dim3(RED_THREADS), 0, s, d_args,
+                       d_nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_capture(const IterArgs & a, float * res_out, double * phi_out, hipStream_t s)
+{
+  hipLaunchKernelGGL(capture_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a, res_out,
+                     phi_out);
+  return DOMA_OK;
+}
+
+int
+launch_targets(const double * phi_g, const double * snaps, float * targets, int N, int count,
+               hipStream_t s)
+{
+  if (count > 0)
+    hipLaunchKernelGGL(target_kernel, dim3(blocks_for((size_t)N * count, 256)), dim3(256), 0, s,
+                       phi_g, snaps, targets, N, count);
   return DOMA_OK;
 }
 

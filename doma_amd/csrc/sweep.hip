@@ -21,10 +21,35 @@ int dispatch_shapes<float, DOMA_SCHEME_TW_DD>(const BrickArgs &, int, int, int, 
 template <>
 int dispatch_shapes<float, DOMA_SCHEME_DD>(const BrickArgs &, int, int, int, int, int,
                                           hipStream_t, int *);
+template <>
+int dispatch_batch_shapes<double, DOMA_SCHEME_TW_DD>(const BatchArgs &, int, int, int,
+                                                    hipStream_t, int *);
+template <>
+int dispatch_batch_shapes<double, DOMA_SCHEME_DD>(const BatchArgs &, int, int, int, hipStream_t,
+                                                 int *);
+template <>
+int dispatch_batch_shapes<float, DOMA_SCHEME_TW_DD>(const BatchArgs &, int, int, int,
+                                                   hipStream_t, int *);
+template <>
+int dispatch_batch_shapes<float, DOMA_SCHEME_DD>(const BatchArgs &, int, int, int, hipStream_t,
+                                                int *);
 } // namespace sweepk
 
 namespace
 {
+int
+dispatch_batch(const BatchArgs & b, int scheme, int precision, int shape, int op, int grid,
+               hipStream_t s, int * per_cu)
+{
+  using namespace sweepk;
+  const bool f32 = precision == DOMA_FP32_SWEEP;
+  if (scheme == DOMA_SCHEME_TW_DD)
+    return f32 ? dispatch_batch_shapes<float, DOMA_SCHEME_TW_DD>(b, shape, op, grid, s, per_cu)
+               : dispatch_batch_shapes<double, DOMA_SCHEME_TW_DD>(b, shape, op, grid, s, per_cu);
+  return f32 ? dispatch_batch_shapes<float, DOMA_SCHEME_DD>(b, shape, op, grid, s, per_cu)
+             : dispatch_batch_shapes<double, DOMA_SCHEME_DD>(b, shape, op, grid, s, per_cu);
+}
+
 int
 dispatch(const BrickArgs & a, int scheme, int precision, int exact, int shape, int op,
          int schedule, int grid, hipStream_t s, int * per_cu)
@@ -112,6 +137,20 @@ sweep_blocks_per_cu(int scheme, int precision, int exact, int shape, int schedul
 {
   BrickArgs dummy{};
   return dispatch(dummy, scheme, precision, exact, shape, 1, schedule, 0, nullptr, per_cu);
+}
+
+int
+launch_brick_batch(const BatchArgs & b, int scheme, int precision, int shape, int grid,
+                   hipStream_t s)
+{
+  return dispatch_batch(b, scheme, precision, shape, 0, grid, s, nullptr);
+}
+
+int
+batch_blocks_per_cu(int scheme, int precision, int shape, int * per_cu)
+{
+  BatchArgs dummy{};
+  return dispatch_batch(dummy, scheme, precision, shape, 1, 0, nullptr, per_cu);
 }
 
 int

@@ -444,26 +444,32 @@ brick_sweep_kernel(BrickArgs a)
 // relaxed device-coherent loads; gives up at a wall-clock deadline, or at once
 // when another wait of this sweep has already expired.
 __device__ __forceinline__ void
-wait_flag(const BrickArgs & a, const uint32_t * f)
+wait_epoch(const uint32_t * f, uint32_t epoch, int * error, long long wait_ticks)
 {
-  if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch)
+  if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch)
     return;
   const long long t0 = wall_clock64();
   int polls = 0;
-  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.epoch)
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch)
   {
     __builtin_amdgcn_s_sleep(2);
     if ((++polls & 31) == 0)
     {
-      if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         return;
-      if (wall_clock64() - t0 > a.wait_ticks)
+      if (wall_clock64() - t0 > wait_ticks)
       {
-        atomicExch(a.error, 1);
+        atomicExch(error, 1);
         return;
       }
     }
   }
+}
+
+__device__ __forceinline__ void
+wait_flag(const BrickArgs & a, const uint32_t * f)
+{
+  wait_epoch(f, a.epoch, a.error, a.wait_ticks);
 }
 
 // One persistent launch per sweep; tasks in ticket (= diagonal) order, task id
@@ -517,6 +523,63 @@ brick_flow_kernel(BrickArgs a)
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Batched persistent launch: the tasks of many 3D problems (same kernel shape
+// and quadrature, any extents) in one ticket order, built on the host so that
+// every task's upwind tasks hold smaller tickets (per problem, diagonal order;
+// across problems, interleaved by diagonal index).  Per-problem arguments are
+// read from a device table by the uniform problem index of the task.
+template <typename R, int SCHEME, int SHAPE>
+__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+brick_batch_kernel(BatchArgs b)
+{
+  constexpr Shape S = kShapes[SHAPE];
+  static_assert(S.dims == 3, "the batched sweep is 3D");
+  using L = Layout<R, 3, S.B, S.DL, S.NDL, S.KO>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int * s_task = reinterpret_cast<int *>(smem + L::TASK_OFF);
+  // Every problem of a batch shares the brick edge and so the plane table.
+  load_plane_table<R, 3, S.B, S.DL, S.NDL, S.KO, S.NT>(b.probs[0], smem);
+  while (true)
+  {
+    __syncthreads();  // see brick_flow_kernel
+    if (threadIdx.x == 0)
+    {
+      const unsigned long long t = atomicAdd(b.ticket, 1ull) - b.ticket_base;
+      *s_task = t < (unsigned long long)b.ntasks ? (int)t : -1;
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(*s_task);
+    if (t < 0)
+      break;
+    const uint32_t code = b.tasks[t];
+    const int p = (int)(code >> 20);
+    const int br = (int)((code >> 6) & (kBatchMaxBricks - 1));
+    const int r = (int)(code & (kBatchMaxPairs - 1));
+    const BrickArgs a = b.probs[p];
+    const int nbricks = a.ntx * a.nty * a.ntz;
+    const int og = r / a.nchunks, ch = r % a.nchunks;
+    const int bi = a.bricks[3 * br + 0], bj = a.bricks[3 * br + 1], bk = a.bricks[3 * br + 2];
+    uint32_t * flags = a.flags + (size_t)r * nbricks;
+    if (threadIdx.x == 0)
+    {
+      if (bi > 0)
+        wait_epoch(flags + (bk * a.nty + bj) * a.ntx + (bi - 1), b.epoch, b.error, b.wait_ticks);
+      if (bj > 0)
+        wait_epoch(flags + (bk * a.nty + (bj - 1)) * a.ntx + bi, b.epoch, b.error, b.wait_ticks);
+      if (bk > 0)
+        wait_epoch(flags + ((bk - 1) * a.nty + bj) * a.ntx + bi, b.epoch, b.error, b.wait_ticks);
+    }
+    __syncthreads();
+    sweep_brick<R, 3, SCHEME, false, S.B, S.DL, S.NDL, S.KO, S.NT, true>(a, bi, bj, bk, og, ch,
+                                                                         smem);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, b.epoch, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -605,6 +668,53 @@ dispatch_seq(const BrickArgs & a, int shape, int op, int schedule, int grid, hip
     return fail(DOMA_ERR_INVALID, "unknown sweep kernel shape");
   return rc;
 }
+// Batched launch (op 0) or occupancy query (op 1) for one shape; 3D shapes only.
+template <typename R, int SCHEME, int SHAPE>
+int
+run_batch_shape(const BatchArgs & b, int op, int grid, hipStream_t s, int * per_cu)
+{
+  if constexpr (kShapes[SHAPE].dims != 3)
+  {
+    return fail(DOMA_ERR_INVALID, "the batched sweep needs a 3D kernel shape");
+  }
+  else
+  {
+    constexpr Shape S = kShapes[SHAPE];
+    constexpr size_t shmem = shmem_bytes<R, 3, SHAPE>();
+    auto k = brick_batch_kernel<R, SCHEME, SHAPE>;
+    static bool ready = false;
+    int rc = prepare(k, shmem, ready);
+    if (rc)
+      return rc;
+    if (op == 1)
+    {
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
+      return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, b);
+    return DOMA_OK;
+  }
+}
+
+template <typename R, int SCHEME>
+int dispatch_batch_shapes(const BatchArgs & b, int shape, int op, int grid, hipStream_t s,
+                          int * per_cu);
+
+template <typename R, int SCHEME, int... IDS>
+int
+dispatch_batch_seq(const BatchArgs & b, int shape, int op, int grid, hipStream_t s, int * per_cu,
+                   std::integer_sequence<int, IDS...>)
+{
+  int rc = DOMA_ERR_INVALID;
+  bool hit = false;
+  (void)((shape == IDS ? (hit = true, rc = run_batch_shape<R, SCHEME, IDS>(b, op, grid, s, per_cu),
+                          true)
+                       : false) ||
+         ...);
+  if (!hit)
+    return fail(DOMA_ERR_INVALID, "unknown sweep kernel shape");
+  return rc;
+}
 } // namespace sweepk
 } // namespace doma
 
@@ -621,6 +731,13 @@ dispatch_seq(const BrickArgs & a, int shape, int op, int schedule, int grid, hip
     using seq = std::make_integer_sequence<int, kNumShapes>;                                       \
     return exact ? dispatch_seq<R, SCHEME, true>(a, shape, op, schedule, grid, s, per_cu, seq{})   \
                  : dispatch_seq<R, SCHEME, false>(a, shape, op, schedule, grid, s, per_cu, seq{}); \
+  }                                                                                                \
+  template <>                                                                                      \
+  int dispatch_batch_shapes<R, SCHEME>(const BatchArgs & b, int shape, int op, int grid,           \
+                                       hipStream_t s, int * per_cu)                                \
+  {                                                                                                \
+    using seq = std::make_integer_sequence<int, kNumShapes>;                                       \
+    return dispatch_batch_seq<R, SCHEME>(b, shape, op, grid, s, per_cu, seq{});                    \
   }                                                                                                \
   }                                                                                                \
   }

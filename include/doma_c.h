@@ -151,6 +151,52 @@ int64_t doma_units_per_sweep(const doma_ctx *ctx);
 int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
 
 /* ------------------------------------------------------------------------- */
+/* Batches: many independent 3D problems, one persistent sweep launch per      */
+/* source-iteration round (no reference counterpart).                          */
+/* ------------------------------------------------------------------------- */
+
+typedef struct doma_batch doma_batch;
+
+typedef struct doma_batch_info {
+  int32_t rounds;      /* batched sweep launches (source-iteration rounds) */
+  int64_t units;       /* cell-direction-group updates swept, all problems */
+  double elapsed_ms;   /* wall time of doma_batch_solve */
+  double sweep_ms;     /* device time of the batched sweep launches */
+  int64_t samples;     /* training snapshots kept */
+} doma_batch_info;
+
+/* count problems; all 3D, same scheme, precision and angular quadrature (any
+ * extents, any groups).  One shared stream; the sweep kernel shape is common to
+ * the batch. */
+int doma_batch_create(int32_t count, const doma_problem_desc *descs, doma_batch **out);
+int doma_batch_destroy(doma_batch *b);
+int doma_batch_set_materials(doma_batch *b, int32_t p, const int32_t *mat, int32_t num_mats,
+                             const double *sigt, const double *sigs, const double *src,
+                             const uint8_t *has_src);
+/* Kernel shape for the batch (schedule is always the persistent dataflow one). */
+int doma_batch_set_tuning(doma_batch *b, const doma_tuning *t);
+int doma_batch_get_tuning(const doma_batch *b, doma_tuning *t);
+/* Times every valid 3D kernel shape on one batched sweep of all problems and
+ * keeps the fastest. */
+int doma_batch_autotune(doma_batch *b, doma_tuning *best, double *best_ms);
+/* n batched sweeps of group 0 of every problem: device ms per launch, units per launch. */
+int doma_batch_time_sweeps(doma_batch *b, int32_t n, double *ms_per_round, int64_t *units);
+/* Every problem solved like doma_solve_fixed_source.  With sample_every > 0,
+ * (residual, converged - current flux) pairs are kept every sample_every
+ * iterations of the first multigroup pass, at most sample_cap per (problem,
+ * group), thinned evenly.  stats: [count] or NULL. */
+int doma_batch_solve(doma_batch *b, double fs_tol, int32_t fs_max_it, double mg_tol,
+                     int32_t mg_max_it, int32_t sample_every, int32_t sample_cap,
+                     doma_stats *stats, doma_batch_info *info);
+int doma_batch_get_flux(doma_batch *b, int32_t p, double *phi);
+/* Snapshots kept for (p, g) and their iteration numbers (iterations may be NULL). */
+int doma_batch_sample_count(const doma_batch *b, int32_t p, int32_t g, int32_t *count,
+                            int32_t *iterations);
+/* Snapshot k of (p, g): residual[N] = latest SI increment, target[N] = phi_conv - phi_k. */
+int doma_batch_get_sample(doma_batch *b, int32_t p, int32_t g, int32_t k, float *residual,
+                          float *target);
+
+/* ------------------------------------------------------------------------- */
 /* Input decks (host C++ front end).                                           */
 /* ------------------------------------------------------------------------- */
 
