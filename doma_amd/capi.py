@@ -59,7 +59,8 @@ class Stats(C.Structure):
         ("success", C.c_int32), ("mg_iterations", C.c_int32), ("mg_residual", C.c_double),
         ("sweeps", C.c_int32), ("si_iterations", C.c_int32 * MAX_GROUPS),
         ("si_residual", C.c_double * MAX_GROUPS), ("elapsed_ms", C.c_double),
-        ("sweep_ms", C.c_double),
+        ("sweep_ms", C.c_double), ("corrections_accepted", C.c_int32),
+        ("corrections_rejected", C.c_int32), ("correction_ms", C.c_double),
     ]
 
 
@@ -117,6 +118,11 @@ SIGNATURES = [
     ("doma_gl_quadrature", C.c_int, [C.c_int32, _dp, _dp]),
     ("doma_media_num_materials", C.c_int32, [C.POINTER(MediaSpec)]),
     ("doma_media_generate", C.c_int, [C.POINTER(MediaSpec), _ip, _dp, _dp, _dp, _u8p]),
+    ("doma_net_param_count", C.c_int64, []),
+    ("doma_set_learned_correction", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64,
+                                              C.c_int32]),
+    ("doma_net_forward", C.c_int, [C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32,
+                                   C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("doma_batch_create", C.c_int, [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("doma_batch_destroy", C.c_int, [C.c_void_p]),
     ("doma_batch_set_materials", C.c_int, [C.c_void_p, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp,
@@ -321,13 +327,7 @@ class Context:
         _check(lib().doma_solve_fixed_source(self.h, fs_tol, fs_max_it, mg_tol, mg_max_it,
                                              _d(phi) if want_flux else None, C.byref(st)),
                "doma_solve_fixed_source")
-        stats = {
-            "success": bool(st.success), "mg_iterations": st.mg_iterations,
-            "mg_residual": st.mg_residual, "sweeps": st.sweeps,
-            "si_iterations": list(st.si_iterations[: self.problem.groups]),
-            "si_residual": list(st.si_residual[: self.problem.groups]),
-            "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms,
-        }
+        stats = _stats_dict(st, self.problem.groups)
         if want_flux:
             phi = phi.reshape(self.problem.groups, self.problem.N)
         return phi, stats
@@ -336,6 +336,18 @@ class Context:
         phi = np.zeros(self.problem.groups * self.problem.N)
         _check(lib().doma_get_flux(self.h, _d(phi)), "doma_get_flux")
         return phi.reshape(self.problem.groups, self.problem.N)
+
+    def set_learned_correction(self, params, every: int = 5):
+        """Enable (every > 0) or disable (every = 0) the learned SI correction."""
+        fp = C.POINTER(C.c_float)
+        if every > 0:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            _check(lib().doma_set_learned_correction(self.h, params.ctypes.data_as(fp),
+                                                     params.size, every),
+                   "doma_set_learned_correction")
+        else:
+            _check(lib().doma_set_learned_correction(self.h, None, 0, 0),
+                   "doma_set_learned_correction")
 
     def time_sweeps(self, g: int, n: int):
         ms, launches = C.c_double(), C.c_int32()
@@ -364,7 +376,26 @@ def _stats_dict(st: Stats, G: int) -> dict:
     return {"success": bool(st.success), "mg_iterations": st.mg_iterations,
             "mg_residual": st.mg_residual, "sweeps": st.sweeps,
             "si_iterations": list(st.si_iterations[:G]), "si_residual": list(st.si_residual[:G]),
-            "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms}
+            "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms,
+            "corrections_accepted": st.corrections_accepted,
+            "corrections_rejected": st.corrections_rejected, "correction_ms": st.correction_ms}
+
+
+def net_param_count() -> int:
+    return int(lib().doma_net_param_count())
+
+
+def net_forward(params: np.ndarray, features: np.ndarray) -> np.ndarray:
+    """Stand-alone GPU forward pass: features [D, H, W, 3] (FP32) -> [D, H, W]."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    D, H, W, _ = f.shape
+    out = np.zeros((D, H, W), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    _check(lib().doma_net_forward(params.ctypes.data_as(fp), params.size, D, H, W,
+                                  f.ctypes.data_as(fp), out.ctypes.data_as(fp)),
+           "doma_net_forward")
+    return out
 
 
 class Batch:

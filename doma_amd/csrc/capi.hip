@@ -443,20 +443,31 @@ check_ctx(const doma_ctx * c, bool need_materials)
 // it == max_it or the residual is <= tol.
 int
 si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, int * conv,
-        double * sweep_ms, int * sweeps)
+        double * sweep_ms, int * sweeps, doma_stats * st = nullptr)
 {
+  // With the learned correction every `every` iterations the network's estimate
+  // of the remaining error is added to the flux and the next sweep starts from
+  // the corrected state; if that sweep's residual is not below the one before
+  // the correction, the correction is undone (the sweep still counts).
+  const bool learned = c->net.ready && c->net.every > 0 && c->dims == 3;
   IterArgs a = iter_args(c, g);
+  if (learned)
+    a.dcur = c->d_dcur;
   int it = 0;
-  double r = 0.0;
+  double r = 0.0, r_before = 0.0;
+  bool pending = false;
   do
   {
+    IterArgs af = a;
+    if (pending)
+      af.corr = c->d_corr;
     DOMA_HIP(hipEventRecord(c->ev0, c->stream));
     int rc = run_sweep(c, g);
     if (rc)
       return rc;
     DOMA_HIP(hipEventRecord(c->ev1, c->stream));
-    if ((rc = launch_finish(a, c->red_blocks, c->stream)) ||
-        (rc = launch_residual_final(a, c->red_blocks, c->stream)))
+    if ((rc = launch_finish(af, c->red_blocks, c->stream)) ||
+        (rc = launch_residual_final(af, c->red_blocks, c->stream)))
       return rc;
     DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 4 * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
@@ -470,8 +481,56 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       *sweep_ms += ms;
     if (sweeps)
       ++*sweeps;
-    r = c->h_res[0];
+    const double rn = c->h_res[0];
     ++it;
+    if (!learned)
+    {
+      r = rn;
+      continue;
+    }
+    if (pending)
+    {
+      pending = false;
+      if (!(rn < r_before))
+      {
+        IterArgs au = a;
+        au.corr = c->d_corr;
+        au.dlast = c->d_dlast;
+        if ((rc = launch_undo_correction(au, c->d_keep, c->stream)))
+          return rc;
+        if (st)
+          ++st->corrections_rejected;
+        r = r_before;
+        continue;
+      }
+      if (st)
+        ++st->corrections_accepted;
+    }
+    std::swap(c->d_dlast, c->d_dcur);
+    a.dcur = c->d_dcur;
+    r = rn;
+    if (tol < r && it < max_it && it % c->net.every == 0)
+    {
+      const double t0 = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now().time_since_epoch())
+                            .count();
+      if ((rc = net::net_apply(c->net, c->d_dlast, c->d_mat, c->d_sigt, c->d_sigs, c->G, g,
+                               c->d_corr, c->stream)))
+        return rc;
+      IterArgs ac = a;
+      ac.corr = c->d_corr;
+      ac.dlast = c->d_dlast;
+      if ((rc = launch_apply_correction(ac, c->d_keep, c->stream)))
+        return rc;
+      DOMA_HIP(hipStreamSynchronize(c->stream));
+      if (st)
+        st->correction_ms += std::chrono::duration<double, std::milli>(
+                                 std::chrono::steady_clock::now().time_since_epoch())
+                                 .count() -
+                             t0;
+      pending = true;
+      r_before = r;
+    }
   } while (it < max_it && tol < r);
   if (its)
     *its = it;
@@ -705,6 +764,11 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_res);
   dev_free(c->d_err);
   dev_free(c->d_ticket);
+  net::net_free(c->net);
+  dev_free(c->d_corr);
+  dev_free(c->d_dlast);
+  dev_free(c->d_dcur);
+  dev_free(c->d_keep);
   if (c->h_res)
     (void)hipHostFree(c->h_res);
   if (c->ev0)
@@ -896,7 +960,7 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
         return rc;
       int its = 0, conv = 0;
       double res = 0.0;
-      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps)))
+      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps, &s)))
         return rc;
       s.si_iterations[g] = its;
       s.si_residual[g] = res;
@@ -1100,4 +1164,76 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
   if (best_ms)
     *best_ms = win_ms;
   return DOMA_OK;
+}
+
+extern "C" int64_t
+doma_net_param_count(void)
+{
+  return net::total_params();
+}
+
+extern "C" int
+doma_set_learned_correction(doma_ctx * c, const float * params, int64_t count, int32_t every)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (every < 0)
+    return fail(DOMA_ERR_INVALID, "doma_set_learned_correction: every must be >= 0");
+  if (every == 0)
+  {
+    net::net_free(c->net);
+    c->net.every = 0;
+    return DOMA_OK;
+  }
+  if (c->dims != 3)
+    return fail(DOMA_ERR_INVALID, "the learned correction is a 3D network");
+  if (!params || count != net::total_params())
+    return fail(DOMA_ERR_INVALID, "doma_set_learned_correction: expected " +
+                                      std::to_string(net::total_params()) + " parameters");
+  if ((rc = net::net_setup(c->net, params, c->nz, c->ny, c->nx)))
+    return rc;
+  const size_t N = (size_t)c->N;
+  if (!c->d_corr &&
+      ((rc = dev_alloc(&c->d_corr, N)) || (rc = dev_alloc(&c->d_dlast, N)) ||
+       (rc = dev_alloc(&c->d_dcur, N)) || (rc = dev_alloc(&c->d_keep, N))))
+    return rc;
+  c->net.every = every;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_net_forward(const float * params, int64_t count, int32_t D, int32_t H, int32_t W,
+                 const float * features, float * out)
+{
+  if (!params || !features || !out || count != net::total_params())
+    return fail(DOMA_ERR_INVALID, "doma_net_forward: bad arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(DOMA_ERR_DEVICE, "no HIP device available");
+  net::Net n;
+  int rc = net::net_setup(n, params, D, H, W);
+  if (rc)
+  {
+    net::net_free(n);
+    return rc;
+  }
+  const size_t V = (size_t)D * H * W;
+  float * df = nullptr;
+  float * dout = nullptr;
+  if (hipMalloc(&df, V * 3 * sizeof(float)) != hipSuccess ||
+      hipMalloc(&dout, V * sizeof(float)) != hipSuccess)
+    rc = fail(DOMA_ERR_DEVICE, "hipMalloc failed");
+  if (!rc && hipMemcpy(df, features, V * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(DOMA_ERR_DEVICE, "upload failed");
+  if (!rc)
+    rc = net::net_forward_features(n, df, dout, nullptr);
+  if (!rc && hipMemcpy(out, dout, V * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(DOMA_ERR_DEVICE, "download failed");
+  if (df)
+    (void)hipFree(df);
+  if (dout)
+    (void)hipFree(dout);
+  net::net_free(n);
+  return rc;
 }

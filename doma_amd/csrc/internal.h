@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "doma_c.h"
+#include "unet.h"
 
 namespace doma
 {
@@ -141,6 +142,9 @@ struct IterArgs
   double * red;            // reduction partials
   double * res;            // [4] residual, diff, total, sweep error flag
   const int * err;         // sweep error flag (bounded wait expired)
+  const double * corr;     // [N] correction applied before this sweep (or null)
+  double * dcur;           // [N] receives this iteration's increment (or null)
+  const double * dlast;    // [N] last accepted increment (correction kernels)
 };
 
 int launch_mg_source(const IterArgs & a, hipStream_t s);
@@ -158,6 +162,10 @@ int launch_batch_residual(const IterArgs * d_args, const int * d_nblocks, int np
 int launch_capture(const IterArgs & a, float * res_out, double * phi_out, hipStream_t s);
 int launch_targets(const double * phi_g, const double * snaps, float * targets, int N, int count,
                    hipStream_t s);
+// Learned correction: keep = phi_g, phi_g += corr, q = S (dlast + corr); undo:
+// phi_g = keep, q = S dlast.
+int launch_apply_correction(const IterArgs & a, double * keep, hipStream_t s);
+int launch_undo_correction(const IterArgs & a, const double * keep, hipStream_t s);
 int launch_set_q(const IterArgs & a, const double * qmg_dev, hipStream_t s);
 int reduction_blocks(int N);
 } // namespace doma
@@ -235,6 +243,13 @@ struct doma_ctx
   uint32_t epoch = 0;
   unsigned long long ticket_base = 0;
   int wall_clock_khz = 100000;
+
+  // Learned SI correction (3D).
+  doma::net::Net net;
+  double * d_corr = nullptr;   // [N]
+  double * d_dlast = nullptr;  // [N] last accepted increment
+  double * d_dcur = nullptr;   // [N] increment of the latest sweep
+  double * d_keep = nullptr;   // [N] flux before the pending correction
 };
 
 namespace doma

@@ -118,6 +118,12 @@ finish_body(const IterArgs & a, int block, int nblocks)
     double d = 0.0;
     for (int s = 0; s < a.ndirsets; ++s)
       d += a.part[(size_t)s * a.N + c];
+    // After a learned correction e the sweep's source was S (d_last + e); the
+    // increment from the corrected flux is K (d_last + e) - e.
+    if (a.corr)
+      d -= a.corr[c];
+    if (a.dcur)
+      a.dcur[c] = d;
     const double v = measure(a, c);
     const size_t pc = (size_t)a.g * a.N + c;
     const double t = a.phi[pc];
@@ -198,6 +204,38 @@ batch_residual_kernel(const IterArgs * args, const int * nblocks)
 {
   const IterArgs a = args[blockIdx.x];
   residual_body(a, nblocks[blockIdx.x]);
+}
+
+// Learned correction of group g: keep phi_g, then phi_g += e and the next
+// source is S (d_last + e).
+__global__ void
+apply_correction_kernel(IterArgs a, double * keep)
+{
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  const int m = a.mat[c];
+  const double sgg = a.sigs[((size_t)m * a.G + a.g) * a.G + a.g];
+  const double e = a.corr[c];
+  const size_t pc = (size_t)a.g * a.N + c;
+  keep[c] = a.phi[pc];
+  a.phi[pc] += e;
+  a.q[c] = norm_term(a.dims, sgg, a.dlast[c] + e);
+}
+
+// Undo a rejected correction: restore the kept flux exactly and the source S d_last.
+__global__ void
+undo_correction_kernel(IterArgs a, const double * keep)
+{
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  const int m = a.mat[c];
+  const double sgg = a.sigs[((size_t)m * a.G + a.g) * a.G + a.g];
+  a.phi[(size_t)a.g * a.N + c] = keep[c];
+  a.q[c] = norm_term(a.dims, sgg, a.dlast[c]);
 }
 
 // Training-data snapshot after a source iteration: the latest increment (the SI
@@ -414,6 +452,22 @@ launch_targets(const double * phi_g, const double * snaps, float * targets, int 
   if (count > 0)
     hipLaunchKernelGGL(target_kernel, dim3(blocks_for((size_t)N * count, 256)), dim3(256), 0, s,
                        phi_g, snaps, targets, N, count);
+  return DOMA_OK;
+}
+
+int
+launch_apply_correction(const IterArgs & a, double * keep, hipStream_t s)
+{
+  hipLaunchKernelGGL(apply_correction_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a,
+                     keep);
+  return DOMA_OK;
+}
+
+int
+launch_undo_correction(const IterArgs & a, const double * keep, hipStream_t s)
+{
+  hipLaunchKernelGGL(undo_correction_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a,
+                     keep);
   return DOMA_OK;
 }
 

@@ -1,0 +1,95 @@
+// unet.h -- the learned source-iteration correction: a 3-level 3D U-Net on
+// BF16 MFMA (FP32 accumulate) that maps the latest SI increment of a group (the
+// SI residual field) plus the group's sigma_t and in-group scattering ratio to an
+// estimate of the remaining error phi_converged - phi.  Host-side description
+// shared by the C ABI, the kernels (unet.hip) and the Python trainer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace doma
+{
+namespace net
+{
+// Channel widths of the three levels and the layer list.  Layers 0-9 are 3x3x3
+// convolutions (zero padding 1) followed by ReLU; layer 10 is the 1x1x1 head.
+constexpr int C1 = 12, C2 = 24, C3 = 32, CIN = 3;
+constexpr int kLayers = 11;
+
+struct LayerSpec
+{
+  int cin_a;  // channels read at full resolution (skip or plain input)
+  int cin_b;  // channels read from the coarser level, upsampled x2 (0: none)
+  int cout;
+  int taps;   // 27 or 1
+  int relu;
+};
+
+// enc1a enc1b | pool | enc2a enc2b | pool | bot_a bot_b | dec2a dec2b | dec1a dec1b | head
+constexpr LayerSpec kSpec[kLayers] = {
+    {CIN, 0, C1, 27, 1}, {C1, 0, C1, 27, 1}, {C1, 0, C2, 27, 1}, {C2, 0, C2, 27, 1},
+    {C2, 0, C3, 27, 1},  {C3, 0, C3, 27, 1}, {C2, C3, C2, 27, 1}, {C2, 0, C2, 27, 1},
+    {C1, C2, C1, 27, 1}, {C1, 0, C1, 27, 1}, {C1, 0, 1, 1, 0},
+};
+
+// Flat FP32 parameter vector: per layer W[cout][tap][cin_a + cin_b] then bias[cout].
+constexpr int64_t layer_params(const LayerSpec & s)
+{
+  return (int64_t)s.cout * s.taps * (s.cin_a + s.cin_b) + s.cout;
+}
+constexpr int64_t total_params()
+{
+  int64_t n = 0;
+  for (const auto & s : kSpec)
+    n += layer_params(s);
+  return n;
+}
+
+constexpr int pad8(int c) { return (c + 7) / 8 * 8; }
+
+// Device-side packed layer: W as BF16 [cout16][kpad], K index = tap * cin_p + ci
+// where cin_p = pad8(cin_a) + pad8(cin_b) (padding slots carry zero weights).
+struct PackedLayer
+{
+  uint16_t * w = nullptr;
+  float * bias = nullptr;
+  int cin_a_p = 0, cin_b_p = 0, cout = 0, cout16 = 0, kpad = 0, taps = 27, relu = 1;
+};
+
+// Activation buffers of one network instance (BF16, channels-last, channel
+// stride padded to 8).
+struct Net
+{
+  bool ready = false;
+  int every = 0;  // apply every `every` source iterations (0: off)
+  int D = 0, H = 0, W = 0;
+  PackedLayer layer[kLayers];
+  uint16_t * feat = nullptr;  // [V][8]
+  uint16_t * e1a = nullptr, * e1b = nullptr;  // [V][16]
+  uint16_t * p1 = nullptr;                    // [V/8][16]
+  uint16_t * e2a = nullptr, * e2b = nullptr;  // [V/8][24]
+  uint16_t * p2 = nullptr;                    // [V/64][24]
+  uint16_t * ba = nullptr, * bb = nullptr;    // [V/64][32]
+  uint16_t * d2a = nullptr, * d2b = nullptr;  // [V/8][24]
+  uint16_t * d1a = nullptr, * d1b = nullptr;  // [V][16]
+  double * red = nullptr;                     // reduction partials
+  double * scale = nullptr;                   // [1] rms of the input increment
+};
+
+// Packs the flat FP32 parameters into per-layer BF16 device layouts and sizes
+// the activation buffers for a D x H x W grid (each divisible by 4).
+int net_setup(Net & n, const float * params, int D, int H, int W);
+void net_free(Net & n);
+
+// corr[c] = U-Net(dlast / s, log sigma_t, log 1/(1 - c_gg)) * s for group g,
+// s = rms(dlast).  mat/sigt/sigs as in IterArgs.
+int net_apply(Net & n, const double * dlast, const int32_t * mat, const double * sigt,
+              const double * sigs, int G, int g, double * corr, hipStream_t s);
+
+// Forward pass on given features (testing): feat FP32 [V][3] -> out FP32 [V].
+int net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream_t s);
+} // namespace net
+} // namespace doma

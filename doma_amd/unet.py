@@ -1,0 +1,238 @@
+"""The learned source-iteration correction: network definition, parameter
+layout, input features, and the trainer.
+
+The product path evaluates the network with the hand-written BF16 MFMA kernels
+of the native library (``doma_set_learned_correction``); this module is the
+tooling around it.  ``UNet`` is the same architecture in PyTorch, used to train
+the weights (autograd, FP32) on the (residual, converged - current flux) pairs
+the batched solver records, and as the FP32 reference that the native kernels
+are tested against.  Parameters cross the boundary as one flat FP32 vector,
+per layer W[cout][tap][cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then
+bias[cout], in the layer order below.
+
+Input channels, per cell, for group g: the latest SI increment divided by its
+rms over the grid, log sigma_t, and log(1 / (1 - c + 1e-4)) with c the in-group
+scattering ratio sigma_s,gg / sigma_t.  Output: the remaining error
+phi_converged - phi in units of the same rms.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+C1, C2, C3, CIN = 12, 24, 32, 3
+# (cin_a, cin_b, cout, taps): cin_b channels come from the coarser level, upsampled.
+SPEC = [(CIN, 0, C1, 27), (C1, 0, C1, 27), (C1, 0, C2, 27), (C2, 0, C2, 27), (C2, 0, C3, 27),
+        (C3, 0, C3, 27), (C2, C3, C2, 27), (C2, 0, C2, 27), (C1, C2, C1, 27), (C1, 0, C1, 27),
+        (C1, 0, 1, 1)]
+
+
+def param_count() -> int:
+    return sum(co * t * (a + b) + co for a, b, co, t in SPEC)
+
+
+def features(dlast, sigt, cgg):
+    """Network input [D, H, W, 3] from the increment and the group's cross sections."""
+    d = np.asarray(dlast, dtype=np.float64)
+    s = math.sqrt(float((d * d).mean())) or 1.0
+    f = np.stack([d / s, np.log(np.maximum(sigt, 1e-6)), np.log(1.0 / (1.0 - cgg + 1e-4))],
+                 axis=-1)
+    return f.astype(np.float32), s
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def make_model():
+    torch = _torch()
+    nn = torch.nn
+    F = torch.nn.functional
+
+    class UNet(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.convs = nn.ModuleList(
+                [nn.Conv3d(a + b, co, 3 if t == 27 else 1, padding=1 if t == 27 else 0)
+                 for a, b, co, t in SPEC])
+
+        def forward(self, x):
+            c = self.convs
+            r = F.relu
+            e1 = r(c[1](r(c[0](x))))
+            e2 = r(c[3](r(c[2](F.avg_pool3d(e1, 2)))))
+            b = r(c[5](r(c[4](F.avg_pool3d(e2, 2)))))
+            d2 = torch.cat([e2, F.interpolate(b, scale_factor=2, mode="nearest")], dim=1)
+            d2 = r(c[7](r(c[6](d2))))
+            d1 = torch.cat([e1, F.interpolate(d2, scale_factor=2, mode="nearest")], dim=1)
+            d1 = r(c[9](r(c[8](d1))))
+            return c[10](d1)
+
+    return UNet()
+
+
+def flatten(model) -> np.ndarray:
+    out = []
+    for conv in model.convs:
+        w = conv.weight.detach().cpu().float()          # [co][ci][kd][kh][kw]
+        co, ci = w.shape[0], w.shape[1]
+        out.append(w.permute(0, 2, 3, 4, 1).reshape(co, -1, ci).reshape(-1).numpy())
+        out.append(conv.bias.detach().cpu().float().numpy())
+    flat = np.concatenate(out).astype(np.float32)
+    assert flat.size == param_count()
+    return flat
+
+
+def unflatten(model, flat):
+    torch = _torch()
+    off = 0
+    with torch.no_grad():
+        for conv, (a, b, co, t) in zip(model.convs, SPEC):
+            ci = a + b
+            k = 3 if t == 27 else 1
+            n = co * t * ci
+            w = torch.from_numpy(np.asarray(flat[off:off + n], dtype=np.float32))
+            conv.weight.copy_(w.reshape(co, k, k, k, ci).permute(0, 4, 1, 2, 3))
+            off += n
+            conv.bias.copy_(torch.from_numpy(np.asarray(flat[off:off + co], dtype=np.float32)))
+            off += co
+    assert off == param_count()
+    return model
+
+
+def bf16_round(flat):
+    """Round FP32 values to BF16 (nearest even), as the native kernels store them."""
+    u = np.asarray(flat, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def collect_samples(batch, problems, max_per_group=None):
+    """(features [D,H,W,3], target/scale [D,H,W]) pairs from a solved Batch."""
+    out = []
+    for i, p in enumerate(problems):
+        n = p.nx
+        for g in range(p.groups):
+            its = batch.sample_iterations(i, g)
+            ks = range(len(its)) if max_per_group is None else range(min(len(its), max_per_group))
+            st = p.sigt[p.mat, g]
+            cgg = np.where(st > 0, p.sigs[p.mat, g, g] / np.where(st > 0, st, 1.0), 0.0)
+            for k in ks:
+                r, t = batch.get_sample(i, g, k)
+                f, s = features(r.reshape(n, n, n), st.reshape(n, n, n), cgg.reshape(n, n, n))
+                out.append((f, (t.reshape(n, n, n) / s).astype(np.float32)))
+    return out
+
+
+def train(samples, steps=2000, crop=32, batch_size=8, lr=2e-3, seed=0, device="cuda", log=None):
+    """Adam on the relative L2 loss over random crops; returns the flat parameters."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    torch.manual_seed(seed)
+    model = make_model().to(device)
+    opt = torch.optim.Adam(model.parameters(), lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
+    xs = [torch.from_numpy(f).permute(3, 0, 1, 2).contiguous() for f, _ in samples]
+    ys = [torch.from_numpy(t)[None].contiguous() for _, t in samples]
+    for step in range(steps):
+        xb, yb = [], []
+        for _ in range(batch_size):
+            j = int(rng.integers(len(xs)))
+            n = xs[j].shape[1]
+            c = min(crop, n)
+            o = [int(rng.integers(0, n - c + 1)) for _ in range(3)]
+            xb.append(xs[j][:, o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c])
+            yb.append(ys[j][:, o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c])
+        if len({x.shape for x in xb}) > 1:
+            continue
+        x = torch.stack(xb).to(device)
+        y = torch.stack(yb).to(device)
+        pred = model(x)
+        num = ((pred - y) ** 2).flatten(1).sum(1)
+        den = (y ** 2).flatten(1).sum(1) + 1e-12
+        loss = (num / den).mean()
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        sched.step()
+        if log and (step % 200 == 0 or step == steps - 1):
+            log(f"train step {step} loss {loss.item():.4f}")
+    return flatten(model.cpu())
+
+
+def training_problems(seed=20261210, small=False):
+    """The batched data-generation set: uneven cubes with 1/2/4 groups and
+    c ~ U[0.9, 0.99] (down-scatter), 8-group full-matrix high-scattering media
+    (c ~ U[0.95, 0.995], log-uniform sigma_t in [0.5, 10]), and one-group media
+    whose in-group scattering ratio reaches 0.995 (slow, diffusive SI)."""
+    from . import capi, workloads
+    from .capi import MediaSpec
+
+    k = 4 if small else 1
+    probs = workloads.batched_uneven(128 // k, seed=seed, sizes=(16, 24, 32, 48))
+    rng = np.random.default_rng(seed + 1)
+    for b in range(32 // k):
+        spec = MediaSpec(n=32, block=8, groups=8, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
+                         c_lo=0.95, c_hi=0.995, scatter_rule=capi.SCATTER_FULL,
+                         source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
+                         source_fraction=0.05, cell_width=1.0, seed=seed * 7 + b)
+        probs.append(workloads.block_media_problem(32, 8, 4, 4, capi.SCHEME_TW_DD, spec,
+                                                   f"full8_{b}"))
+    for b in range(32 // k):
+        n = int(rng.choice([24, 32, 48]))
+        lo = float(rng.choice([0.5, 0.8, 0.95]))
+        spec = MediaSpec(n=n, block=8, groups=1, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
+                         c_lo=lo, c_hi=0.995, scatter_rule=capi.SCATTER_DOWN1,
+                         source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
+                         source_fraction=0.1, cell_width=1.0, seed=seed * 11 + b)
+        # one group: the whole scattering ratio stays in-group
+        probs.append(workloads.block_media_problem(n, 1, 4, 4, capi.SCHEME_TW_DD, spec,
+                                                   f"diffusive_{b}"))
+    return probs
+
+
+def main(argv=None):
+    """Generate training data with the batched solver and train the correction:
+    python -m doma_amd.unet OUT.npy [--steps N] [--small]."""
+    import argparse
+    import sys
+    import time
+
+    from .capi import Batch
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("out")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--small", action="store_true")
+    ap.add_argument("--sample-cap", type=int, default=8)
+    args = ap.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    probs = training_problems(small=args.small)
+    # one batch per quadrature (the batch shares one)
+    groups = {}
+    for p in probs:
+        groups.setdefault(p.dirs, []).append(p)
+    samples = []
+    for dirs, ps in groups.items():
+        log(f"data: batch of {len(ps)} problems, {dirs} directions per octant")
+        with Batch(ps) as b:
+            b.autotune()
+            stats, info = b.solve(1e-6, 5000, 1e-6, 60, sample_every=5,
+                                  sample_cap=args.sample_cap)
+            log(f"data: {info['rounds']} rounds, {info['samples']} samples, "
+                f"{info['elapsed_ms'] / 1e3:.1f} s")
+            samples += collect_samples(b, ps)
+    log(f"data: {len(samples)} pairs in {time.time() - t0:.0f} s; training")
+    flat = train(samples, steps=args.steps, log=log)
+    np.save(args.out, flat)
+    log(f"saved {args.out} ({flat.size} parameters) after {time.time() - t0:.0f} s")
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,9 @@ typedef struct doma_stats {
   double si_residual[DOMA_MAX_GROUPS];
   double elapsed_ms;             /* wall time of the call on the context stream */
   double sweep_ms;               /* time spent in sweep kernels (HIP events) */
+  int32_t corrections_accepted;  /* learned corrections kept (doma_set_learned_correction) */
+  int32_t corrections_rejected;  /* learned corrections undone because the residual rose */
+  double correction_ms;          /* wall time spent evaluating the correction network */
 } doma_stats;
 
 const char *doma_last_error(void);
@@ -149,6 +152,27 @@ int64_t doma_units_per_sweep(const doma_ctx *ctx);
 /* Project-written autotuner: times the candidate kernel shapes on this
  * context's problem and keeps the fastest (cached per problem shape). */
 int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
+
+/* ------------------------------------------------------------------------- */
+/* Learned source-iteration correction (no reference counterpart; replaces the  */
+/* reference's hand-built acceleration, TransportSolver1D.cpp:587-800, in 3D). */
+/* ------------------------------------------------------------------------- */
+
+/* Number of FP32 parameters of the correction network (a 3-level 3D U-Net,
+ * channels 12/24/32): per layer W[cout][tap][cin] then bias[cout]. */
+int64_t doma_net_param_count(void);
+
+/* Enable the correction on a 3D context (extents divisible by 4): every `every`
+ * source iterations the network maps the latest increment, sigma_t and the
+ * in-group scattering ratio of the group to a flux correction (BF16 on MFMA,
+ * FP32 accumulate); the correction is undone if the next residual is not below
+ * the one before it.  every = 0 disables it; params may then be NULL. */
+int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t count, int32_t every);
+
+/* Stand-alone forward pass (diagnostics and tests): features[V][3] (FP32, V =
+ * D*H*W, x fastest) -> out[V]. */
+int doma_net_forward(const float *params, int64_t count, int32_t D, int32_t H, int32_t W,
+                     const float *features, float *out);
 
 /* ------------------------------------------------------------------------- */
 /* Batches: many independent 3D problems, one persistent sweep launch per      */

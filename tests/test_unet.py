@@ -1,0 +1,130 @@
+"""The learned SI correction: the hand-written BF16 MFMA U-Net kernels against
+the same network in PyTorch FP32, and the correction's plumbing in source
+iteration (a zero network must leave SI bitwise unchanged; a useless network
+must be rejected and still converge to the same answer)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+import doma_amd
+from doma_amd import capi, unet, workloads
+
+
+def test_param_layout_host():
+    assert unet.param_count() == capi.net_param_count() == 144185
+    torch = pytest.importorskip("torch")
+    m = unet.make_model()
+    m2 = unet.unflatten(unet.make_model(), unet.flatten(m))
+    for a, b in zip(m.state_dict().values(), m2.state_dict().values()):
+        assert torch.equal(a, b)
+
+
+def _torch_forward(flat, feats):
+    import torch
+    m = unet.unflatten(unet.make_model(), flat)
+    x = torch.from_numpy(feats).permute(3, 0, 1, 2)[None]
+    with torch.no_grad():
+        return m(x)[0, 0].numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(16, 20, 24), (32, 32, 32), (8, 12, 16)])
+def test_net_forward_matches_torch(dims):
+    """BF16 activations and weights, FP32 accumulation: relative L2 error of the
+    output against FP32 PyTorch with the same (BF16-rounded) weights < 3e-2."""
+    torch = pytest.importorskip("torch")
+    torch.manual_seed(1)
+    flat = unet.bf16_round(unet.flatten(unet.make_model()))
+    rng = np.random.default_rng(2)
+    D, H, W = dims
+    f = rng.standard_normal((D, H, W, 3)).astype(np.float32)
+    print("net forward", dims, flush=True)
+    got = capi.net_forward(flat, f)
+    want = _torch_forward(flat, f)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < 3e-2, err
+    # a layout mistake (axis swap, wrong tap) is an O(1) error, far above this
+    assert np.abs(got - want).max() < 0.15 * np.abs(want).max()
+
+
+@pytest.mark.gpu
+def test_net_single_tap_exact():
+    """One non-zero weight per layer path: with weights and inputs exactly
+    representable in BF16 the output is exact, pinning tap and channel order."""
+    flat = np.zeros(unet.param_count(), dtype=np.float32)
+    torch = pytest.importorskip("torch")
+    m = unet.make_model()
+    with torch.no_grad():
+        for conv in m.convs:
+            conv.weight.zero_()
+            conv.bias.zero_()
+        # enc1a: out0 <- in channel 1 at offset (dz, dy, dx) = (+1, 0, -1)
+        m.convs[0].weight[0, 1, 2, 1, 0] = 1.0
+        for l in (1, 2, 3, 4, 5, 7, 9):
+            m.convs[l].weight[0, 0, 1, 1, 1] = 1.0
+        m.convs[6].weight[0, 0, 1, 1, 1] = 0.5       # skip path
+        m.convs[6].weight[0, 24, 1, 1, 1] = 0.25     # upsampled path
+        m.convs[8].weight[0, 0, 1, 1, 1] = 1.0
+        m.convs[8].weight[0, 12, 0, 1, 2] = 0.5      # upsampled, offset (-1, 0, +1)
+        m.convs[10].weight[0, 0] = 2.0
+        m.convs[10].bias[0] = 0.125
+    flat = unet.flatten(m)
+    rng = np.random.default_rng(3)
+    f = np.zeros((16, 16, 16, 3), dtype=np.float32)
+    f[..., 1] = rng.integers(0, 8, size=(16, 16, 16)) / 8.0
+    got = capi.net_forward(flat, f)
+    want = _torch_forward(flat, f)
+    assert np.allclose(got, want, rtol=1e-2, atol=1e-2)
+
+
+def _slow_problem(n=16, seed=5):
+    spec = capi.MediaSpec(n=n, block=8, groups=1, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
+                          c_lo=0.95, c_hi=0.995, scatter_rule=capi.SCATTER_DOWN1,
+                          source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
+                          source_fraction=0.2, cell_width=1.0, seed=seed)
+    return workloads.block_media_problem(n, 1, 2, 2, capi.SCHEME_TW_DD, spec, "slow16")
+
+
+@pytest.mark.gpu
+def test_zero_network_is_plain_si():
+    p = _slow_problem()
+    with doma_amd.Context(p) as ctx:
+        phi0, st0 = ctx.solve_fixed_source(1e-8, 3000, 1e-8, 10)
+        ctx.set_learned_correction(np.zeros(unet.param_count(), np.float32), every=5)
+        phi1, st1 = ctx.solve_fixed_source(1e-8, 3000, 1e-8, 10)
+    assert st0["si_iterations"] == st1["si_iterations"]
+    assert np.array_equal(phi0, phi1)
+    assert st1["corrections_accepted"] + st1["corrections_rejected"] > 0
+
+
+@pytest.mark.gpu
+def test_bad_network_is_rejected_and_converges():
+    torch = pytest.importorskip("torch")
+    torch.manual_seed(4)
+    flat = unet.flatten(unet.make_model()) * 50.0  # large random corrections
+    p = _slow_problem(seed=6)
+    with doma_amd.Context(p) as ctx:
+        phi0, st0 = ctx.solve_fixed_source(1e-9, 5000, 1e-9, 10)
+        ctx.set_learned_correction(flat, every=5)
+        phi1, st1 = ctx.solve_fixed_source(1e-9, 5000, 1e-9, 10)
+    print("bad net", st1["corrections_accepted"], st1["corrections_rejected"], flush=True)
+    assert st1["success"] and st1["corrections_rejected"] > 0
+    if st1["corrections_accepted"] == 0:
+        # every correction undone exactly: the plain iterates, bitwise
+        assert np.array_equal(phi1, phi0)
+    assert rel_err(phi1, phi0).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_correction_rejects_wrong_inputs():
+    p = workloads.random_blocks(16, 1, 2, 2)
+    with doma_amd.Context(p) as ctx:
+        with pytest.raises(doma_amd.DomaError):
+            ctx.set_learned_correction(np.zeros(10, np.float32), every=5)
+    p2 = workloads.random_blocks(18, 1, 2, 2)  # 18 is not divisible by 4
+    with doma_amd.Context(p2) as ctx:
+        with pytest.raises(doma_amd.DomaError):
+            ctx.set_learned_correction(np.zeros(unet.param_count(), np.float32), every=5)
