@@ -128,6 +128,83 @@ def batch_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def learned_mode(args):
+    """Learned-corrected FP32-sweep SI against plain FP64 GPU SI on the same
+    high-scattering problem, both to a 1e-6 residual, and both against plain
+    FP64 SI converged to 1e-9 (no corrections).  With --dd the cells use
+    diamond difference, whose source iteration is linear and has one fixed point;
+    the reference's increment-form TW-DD is not additive (its weights depend on
+    each increment, and switch to DD below 1e-8), so its converged flux depends
+    on the iteration path and a corrected path ends at a slightly different one."""
+    import numpy as np
+
+    from doma_amd import workloads
+    from doma_amd.capi import FP32_SWEEP, FP64, Context
+
+    weights = np.load(ROOT / "doma_amd" / "weights" / "si_correction_unet.npy")
+    if args.workload == "diffusive":
+        p = workloads.diffusive(args.n)
+    else:
+        p = workloads.high_scatter(args.n, args.groups)
+    if args.dd:
+        from doma_amd.capi import SCHEME_DD
+        p.scheme = SCHEME_DD
+        p.name += "_dd"
+    tol, fs_max, mg_max = 1e-6, 50000, 500
+    log(f"bench: {p.name} N={p.N} groups={p.groups} directions={p.noct * p.dirs}")
+
+    def run(precision, learned, tol_, label, warm=False):
+        ctx = Context(p, precision=precision)
+        ctx.autotune(0)
+        if learned:
+            ctx.set_learned_correction(weights, every=args.every)
+        if warm:
+            ctx.set_warm_start(True)
+        t0 = time.perf_counter()
+        phi, st = ctx.solve_fixed_source(tol_, fs_max, tol_, mg_max)
+        wall = time.perf_counter() - t0
+        ctx.close()
+        log(f"bench: {label}: {wall:.2f} s, {st['sweeps']} sweeps, MG {st['mg_iterations']}, "
+            f"corrections {st['corrections_accepted']}+/{st['corrections_rejected']}-")
+        return phi, st, wall
+
+    phi_p, st_p, t_p = run(FP64, False, tol, "plain FP64")
+    phi_l, st_l, t_l = run(FP32_SWEEP, True, tol, "learned FP32")
+    phi_w, st_w, t_w = (None, None, None)
+    if p.groups > 1:
+        phi_w, st_w, t_w = run(FP32_SWEEP, True, tol, "learned FP32 + warm start", warm=True)
+    out = {"workload": p.name, "tolerance": tol, "every": args.every,
+           "plain_fp64": {"wall_s": round(t_p, 3), "sweeps": st_p["sweeps"],
+                          "mg_iterations": st_p["mg_iterations"], "success": st_p["success"]},
+           "learned_fp32": {"wall_s": round(t_l, 3), "sweeps": st_l["sweeps"],
+                            "mg_iterations": st_l["mg_iterations"], "success": st_l["success"],
+                            "corrections_accepted": st_l["corrections_accepted"],
+                            "corrections_rejected": st_l["corrections_rejected"],
+                            "correction_s": round(st_l["correction_ms"] / 1e3, 3)},
+           "speedup": round(t_p / t_l, 3)}
+    if st_w is not None:
+        out["learned_fp32_warm"] = {
+            "wall_s": round(t_w, 3), "sweeps": st_w["sweeps"],
+            "mg_iterations": st_w["mg_iterations"], "success": st_w["success"],
+            "corrections_accepted": st_w["corrections_accepted"],
+            "corrections_rejected": st_w["corrections_rejected"]}
+        out["speedup_warm"] = round(t_p / t_w, 3)
+    if not args.no_reference:
+        phi_r, st_r, t_r = run(FP64, False, 1e-9, "reference: plain FP64 to 1e-9")
+        def errs(phi):
+            d = np.abs(phi - phi_r)
+            return {"max_rel": [float(x) for x in d.max(axis=1) / np.abs(phi_r).max(axis=1)],
+                    "l2_rel": [float(x) for x in np.linalg.norm(phi - phi_r, axis=1) /
+                               np.linalg.norm(phi_r, axis=1)]}
+        out["reference"] = {"tolerance": 1e-9, "wall_s": round(t_r, 3),
+                            "sweeps": st_r["sweeps"]}
+        out["plain_fp64"]["error"] = errs(phi_p)
+        out["learned_fp32"]["error"] = errs(phi_l)
+        if st_w is not None:
+            out["learned_fp32_warm"]["error"] = errs(phi_w)
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(nbytes=256 << 20, reps=3):
     """A plain device-to-device copy of known size, for scaling HBM counters."""
     import torch
@@ -150,11 +227,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
-    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch"], default="solve")
+    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned"],
+                    default="solve")
+    ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
+    ap.add_argument("--no-reference", action="store_true")
+    ap.add_argument("--dd", action="store_true", help="diamond difference (learned mode)")
+    ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
     ap.add_argument("--count", type=int, default=512, help="problems (batch mode)")
     ap.add_argument("--sample-cap", type=int, default=8, help="snapshots per problem and group")
-    ap.add_argument("--workload", choices=["random_blocks", "multigroup_full"],
-                    default="random_blocks")
+    ap.add_argument("--workload", choices=["random_blocks", "multigroup_full", "high_scatter",
+                                           "diffusive"], default="random_blocks")
     ap.add_argument("--shape", type=int, default=0, help="kernel shape id + 1 (sweeps mode)")
     ap.add_argument("--schedule", type=int, default=0)
     ap.add_argument("--n-sweeps", type=int, default=10)
@@ -170,6 +252,8 @@ def main():
         return copy_calibration()
     if args.mode == "batch":
         return batch_mode(args)
+    if args.mode == "learned":
+        return learned_mode(args)
 
     from doma_amd import workloads
     from doma_amd.capi import FP64, Context

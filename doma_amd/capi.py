@@ -121,6 +121,7 @@ SIGNATURES = [
     ("doma_net_param_count", C.c_int64, []),
     ("doma_set_learned_correction", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64,
                                               C.c_int32]),
+    ("doma_set_warm_start", C.c_int, [C.c_void_p, C.c_int32]),
     ("doma_net_forward", C.c_int, [C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32,
                                    C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("doma_batch_create", C.c_int, [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
@@ -336,6 +337,10 @@ class Context:
         phi = np.zeros(self.problem.groups * self.problem.N)
         _check(lib().doma_get_flux(self.h, _d(phi)), "doma_get_flux")
         return phi.reshape(self.problem.groups, self.problem.N)
+
+    def set_warm_start(self, enable: bool = True):
+        """Start each group's iteration from the previous multigroup iterate."""
+        _check(lib().doma_set_warm_start(self.h, int(enable)), "doma_set_warm_start")
 
     def set_learned_correction(self, params, every: int = 5):
         """Enable (every > 0) or disable (every = 0) the learned SI correction."""

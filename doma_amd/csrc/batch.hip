@@ -9,7 +9,8 @@
 // Training data: every `sample_every` source iterations of the first multigroup
 // pass, the latest increment (the SI residual field) and the accumulated flux of
 // the group are kept on the device; when the group's iteration has converged
-// each stored flux becomes the target phi_converged - phi_k (FP32).  At most
+// each stored flux becomes the target phi_converged - phi_k (FP32); snapshots
+// of iterations that did not converge are dropped.  At most
 // `sample_cap` snapshots are kept per (problem, group): when the store is full,
 // every other snapshot is dropped and the stride doubles, so the kept snapshots
 // stay evenly spread over the iteration.
@@ -355,16 +356,22 @@ capture(doma_batch * b, int p, int g, int it, int every)
   return DOMA_OK;
 }
 
+// Targets of (p, g) once its source iteration has ended.  Snapshots of an
+// iteration that did not converge have no valid target and are dropped, and so is
+// a snapshot taken at the converged iterate itself (its target is zero).
 int
-finalize_samples(doma_batch * b, int p, int g)
+finalize_samples(doma_batch * b, int p, int g, bool converged, int final_it)
 {
   SampleSet & s = b->samples[p][g];
   doma_ctx * c = b->ctx[p];
-  if (s.finalized || s.count == 0)
-  {
-    s.finalized = true;
+  s.finalized = true;
+  if (!converged)
+    s.count = 0;
+  if (s.count > 0 && s.its[s.count - 1] == final_it)
+    --s.count;
+  s.its.resize(s.count);
+  if (s.count == 0)
     return DOMA_OK;
-  }
   const size_t N = (size_t)c->N;
   if (!s.d_tgt)
   {
@@ -372,7 +379,6 @@ finalize_samples(doma_batch * b, int p, int g)
     if (rc)
       return rc;
   }
-  s.finalized = true;
   return launch_targets(c->d_phi + (size_t)g * N, s.d_phi, s.d_tgt, c->N, s.count, b->stream);
 }
 } // namespace
@@ -683,7 +689,8 @@ doma_batch_solve(doma_batch * b, double fs_tol, int32_t fs_max_it, double mg_tol
       changed = true;
       s.st.si_iterations[s.g] = s.si_it;
       s.st.si_residual[s.g] = r;
-      if (sample_every > 0 && s.mg_it == 0 && (rc = finalize_samples(b, p, s.g)))
+      if (sample_every > 0 && s.mg_it == 0 &&
+          (rc = finalize_samples(b, p, s.g, s.si_it < fs_max_it, s.si_it)))
         return rc;
       if (s.si_it >= fs_max_it)
       {

@@ -443,7 +443,8 @@ check_ctx(const doma_ctx * c, bool need_materials)
 // it == max_it or the residual is <= tol.
 int
 si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, int * conv,
-        double * sweep_ms, int * sweeps, doma_stats * st = nullptr)
+        double * sweep_ms, int * sweeps, doma_stats * st = nullptr,
+        const double * first_corr = nullptr)
 {
   // With the learned correction every `every` iterations the network's estimate
   // of the remaining error is added to the flux and the next sweep starts from
@@ -456,11 +457,17 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
   int it = 0;
   double r = 0.0, r_before = 0.0;
   bool pending = false;
+  // After a rejected correction the next attempt waits twice as long (up to 16x
+  // `every`), so a network that does not help costs few wasted sweeps.
+  const int every = learned ? c->net.every : 0;
+  int gap = every, next_at = every;
   do
   {
     IterArgs af = a;
     if (pending)
       af.corr = c->d_corr;
+    else if (it == 0 && first_corr)
+      af.corr = first_corr;  // warm start: the first increment is measured from prev_g
     DOMA_HIP(hipEventRecord(c->ev0, c->stream));
     int rc = run_sweep(c, g);
     if (rc)
@@ -501,15 +508,19 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
         if (st)
           ++st->corrections_rejected;
         r = r_before;
+        gap = std::min(2 * gap, 16 * every);
+        next_at = it + gap;
         continue;
       }
       if (st)
         ++st->corrections_accepted;
+      gap = every;
+      next_at = it + gap;
     }
     std::swap(c->d_dlast, c->d_dcur);
     a.dcur = c->d_dcur;
     r = rn;
-    if (tol < r && it < max_it && it % c->net.every == 0)
+    if (tol < r && it < max_it && it >= next_at)
     {
       const double t0 = std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now().time_since_epoch())
@@ -958,9 +969,13 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
       IterArgs a = iter_args(c, g);
       if ((rc = launch_mg_source(a, c->stream)))
         return rc;
+      const bool warm = c->warm_start && mg_it > 0 && c->dims > 1;
+      if (warm && (rc = launch_warm_start(a, c->stream)))
+        return rc;
       int its = 0, conv = 0;
       double res = 0.0;
-      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps, &s)))
+      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps, &s,
+                        warm ? c->d_prev + (size_t)g * c->N : nullptr)))
         return rc;
       s.si_iterations[g] = its;
       s.si_residual[g] = res;
@@ -1236,4 +1251,14 @@ doma_net_forward(const float * params, int64_t count, int32_t D, int32_t H, int3
     (void)hipFree(dout);
   net::net_free(n);
   return rc;
+}
+
+extern "C" int
+doma_set_warm_start(doma_ctx * c, int32_t enable)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  c->warm_start = enable ? 1 : 0;
+  return DOMA_OK;
 }

@@ -166,6 +166,8 @@ int launch_targets(const double * phi_g, const double * snaps, float * targets, 
 // phi_g = keep, q = S dlast.
 int launch_apply_correction(const IterArgs & a, double * keep, hipStream_t s);
 int launch_undo_correction(const IterArgs & a, const double * keep, hipStream_t s);
+// Warm start: phi_g = prev_g, q += S prev_g (2D/3D).
+int launch_warm_start(const IterArgs & a, hipStream_t s);
 int launch_set_q(const IterArgs & a, const double * qmg_dev, hipStream_t s);
 int reduction_blocks(int N);
 } // namespace doma
@@ -250,6 +252,7 @@ struct doma_ctx
   double * d_dlast = nullptr;  // [N] last accepted increment
   double * d_dcur = nullptr;   // [N] increment of the latest sweep
   double * d_keep = nullptr;   // [N] flux before the pending correction
+  int warm_start = 0;          // start group iterations from the previous MG iterate
 };
 
 namespace doma

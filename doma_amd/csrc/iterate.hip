@@ -238,6 +238,24 @@ undo_correction_kernel(IterArgs a, const double * keep)
   a.q[c] = norm_term(a.dims, sgg, a.dlast[c]);
 }
 
+// Warm start of group g from the previous multigroup iterate (prev_g): phi_g =
+// prev_g and the first sweep's source gains S prev_g; that sweep's increment is
+// then its output minus prev_g (finish with corr = prev_g).
+__global__ void
+warm_start_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  const int m = a.mat[c];
+  const double sgg = a.sigs[((size_t)m * a.G + a.g) * a.G + a.g];
+  const size_t pc = (size_t)a.g * a.N + c;
+  const double f = a.prev[pc];
+  a.phi[pc] = f;
+  a.q[c] += norm_term(a.dims, sgg, f);
+}
+
 // Training-data snapshot after a source iteration: the latest increment (the SI
 // residual field, FP32) and the accumulated scalar flux of the group (FP64).
 __global__ void
@@ -468,6 +486,13 @@ launch_undo_correction(const IterArgs & a, const double * keep, hipStream_t s)
 {
   hipLaunchKernelGGL(undo_correction_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a,
                      keep);
+  return DOMA_OK;
+}
+
+int
+launch_warm_start(const IterArgs & a, hipStream_t s)
+{
+  hipLaunchKernelGGL(warm_start_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
   return DOMA_OK;
 }
 

@@ -84,12 +84,13 @@ struct Net
 int net_setup(Net & n, const float * params, int D, int H, int W);
 void net_free(Net & n);
 
-// corr[c] = U-Net(dlast / s, log sigma_t, log 1/(1 - c_gg)) * s for group g,
-// s = rms(dlast).  mat/sigt/sigs as in IterArgs.
+// corr[c] = s * U-Net(dlast / s, log sigma_t, log 1/(1 - c_gg + 1e-4)) /
+// (1 - c_gg + 1e-4) for group g, s = rms(dlast).  mat/sigt/sigs as in IterArgs.
 int net_apply(Net & n, const double * dlast, const int32_t * mat, const double * sigt,
               const double * sigs, int G, int g, double * corr, hipStream_t s);
 
-// Forward pass on given features (testing): feat FP32 [V][3] -> out FP32 [V].
+// Forward pass on given features (testing): feat FP32 [V][3] -> raw network
+// output FP32 [V] (no scale, no gain).
 int net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream_t s);
 } // namespace net
 } // namespace doma

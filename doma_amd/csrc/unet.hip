@@ -248,19 +248,39 @@ features_f32_kernel(const float * in, int N, uint16_t * feat)
     feat[(size_t)c * 8 + k] = k < 3 ? f2bf(in[(size_t)c * 3 + k]) : 0;
 }
 
-// 1x1x1 head: out = scale * (w . h + b).
-template <typename T>
+// 1x1x1 head: y = w . h + b.  Raw output for the stand-alone forward pass.
+__device__ inline float
+head_value(const uint16_t * h, int stride, int C, const float * wb, int c)
+{
+  float s = wb[C];
+  for (int k = 0; k < C; ++k)
+    s += wb[k] * bf2f(h[(size_t)c * stride + k]);
+  return s;
+}
+
 __global__ void
-head_kernel(const uint16_t * h, int stride, int C, const float * wb, const double * scale, int N,
-            T * out)
+head_raw_kernel(const uint16_t * h, int stride, int C, const float * wb, int N, float * out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < N)
+    out[c] = head_value(h, stride, C, wb, c);
+}
+
+// Correction: e = scale * y / (1 - c_gg + 1e-4).  The network predicts the error
+// in units of the infinite-medium gain, which keeps its output O(1) from weakly
+// to strongly scattering cells.
+__global__ void
+head_corr_kernel(const uint16_t * h, int stride, int C, const float * wb, const double * scale,
+                 const int32_t * mat, const double * sigt, const double * sigs, int G, int g,
+                 int N, double * out)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N)
     return;
-  float s = wb[C];
-  for (int k = 0; k < C; ++k)
-    s += wb[k] * bf2f(h[(size_t)c * stride + k]);
-  out[c] = scale ? (T)(scale[0] * (double)s) : (T)s;
+  const int m = mat[c];
+  const double st = sigt[(size_t)m * G + g];
+  const double cgg = st > 0.0 ? sigs[((size_t)m * G + g) * G + g] / st : 0.0;
+  out[c] = scale[0] * (double)head_value(h, stride, C, wb, c) / (1.0 - cgg + 1e-4);
 }
 
 inline unsigned
@@ -464,8 +484,8 @@ net_apply(Net & n, const double * dlast, const int32_t * mat, const double * sig
   int rc = body(n, s);
   if (rc)
     return rc;
-  hipLaunchKernelGGL(head_kernel<double>, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1),
-                     C1, n.layer[10].bias, n.scale, N, corr);
+  hipLaunchKernelGGL(head_corr_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
+                     n.layer[10].bias, n.scale, mat, sigt, sigs, G, g, N, corr);
   return DOMA_OK;
 }
 
@@ -480,8 +500,8 @@ net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream
   int rc = body(n, s);
   if (rc)
     return rc;
-  hipLaunchKernelGGL(head_kernel<float>, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1),
-                     C1, n.layer[10].bias, (const double *)nullptr, N, out_dev);
+  hipLaunchKernelGGL(head_raw_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
+                     n.layer[10].bias, N, out_dev);
   return DOMA_OK;
 }
 } // namespace net

@@ -11,9 +11,11 @@ per layer W[cout][tap][cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then
 bias[cout], in the layer order below.
 
 Input channels, per cell, for group g: the latest SI increment divided by its
-rms over the grid, log sigma_t, and log(1 / (1 - c + 1e-4)) with c the in-group
-scattering ratio sigma_s,gg / sigma_t.  Output: the remaining error
-phi_converged - phi in units of the same rms.
+rms s over the grid, log sigma_t, and log(1 / (1 - c + 1e-4)) with c the
+in-group scattering ratio sigma_s,gg / sigma_t.  Output y: the remaining error
+phi_converged - phi = s * y / (1 - c + 1e-4), i.e. in units of the increment
+times the infinite-medium gain, which keeps y O(1) from weakly to strongly
+scattering cells.
 """
 from __future__ import annotations
 
@@ -110,7 +112,7 @@ def bf16_round(flat):
 
 
 def collect_samples(batch, problems, max_per_group=None):
-    """(features [D,H,W,3], target/scale [D,H,W]) pairs from a solved Batch."""
+    """(features [D,H,W,3], scaled target [D,H,W]) pairs from a solved Batch."""
     out = []
     for i, p in enumerate(problems):
         n = p.nx
@@ -121,45 +123,63 @@ def collect_samples(batch, problems, max_per_group=None):
             cgg = np.where(st > 0, p.sigs[p.mat, g, g] / np.where(st > 0, st, 1.0), 0.0)
             for k in ks:
                 r, t = batch.get_sample(i, g, k)
-                f, s = features(r.reshape(n, n, n), st.reshape(n, n, n), cgg.reshape(n, n, n))
-                out.append((f, (t.reshape(n, n, n) / s).astype(np.float32)))
+                c3 = cgg.reshape(n, n, n)
+                f, s = features(r.reshape(n, n, n), st.reshape(n, n, n), c3)
+                y = t.reshape(n, n, n) * (1.0 - c3 + 1e-4) / s
+                out.append((f, y.astype(np.float32)))
     return out
 
 
-def train(samples, steps=2000, crop=32, batch_size=8, lr=2e-3, seed=0, device="cuda", log=None):
-    """Adam on the relative L2 loss over random crops; returns the flat parameters."""
+def train(samples, steps=2000, crops=(16, 32), batch_size=8, lr=1e-3, seed=0, device="cuda",
+          log=None, holdout=0.05):
+    """Adam on the relative L2 loss over random cubic crops (edge drawn from
+    `crops` per step, samples at least that large); returns the flat parameters
+    and the mean relative L2 error on held-out whole samples."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     torch.manual_seed(seed)
+    order = rng.permutation(len(samples))
+    nh = max(1, int(holdout * len(samples)))
+    held = [samples[i] for i in order[:nh]]
+    fit = [samples[i] for i in order[nh:]]
     model = make_model().to(device)
     opt = torch.optim.Adam(model.parameters(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
-    xs = [torch.from_numpy(f).permute(3, 0, 1, 2).contiguous() for f, _ in samples]
-    ys = [torch.from_numpy(t)[None].contiguous() for _, t in samples]
+    xs = [torch.from_numpy(f).permute(3, 0, 1, 2).contiguous() for f, _ in fit]
+    ys = [torch.from_numpy(t)[None].contiguous() for _, t in fit]
+    sizes = np.array([x.shape[1] for x in xs])
     for step in range(steps):
+        c = int(rng.choice(crops))
+        cand = np.nonzero(sizes >= c)[0]
+        if cand.size == 0:
+            continue
         xb, yb = [], []
         for _ in range(batch_size):
-            j = int(rng.integers(len(xs)))
+            j = int(cand[rng.integers(cand.size)])
             n = xs[j].shape[1]
-            c = min(crop, n)
             o = [int(rng.integers(0, n - c + 1)) for _ in range(3)]
             xb.append(xs[j][:, o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c])
             yb.append(ys[j][:, o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c])
-        if len({x.shape for x in xb}) > 1:
-            continue
         x = torch.stack(xb).to(device)
         y = torch.stack(yb).to(device)
         pred = model(x)
-        num = ((pred - y) ** 2).flatten(1).sum(1)
-        den = (y ** 2).flatten(1).sum(1) + 1e-12
-        loss = (num / den).mean()
+        # relative L2 over the whole batch (no per-crop ratios of tiny norms)
+        loss = ((pred - y) ** 2).sum() / ((y ** 2).sum() + 1e-12)
         opt.zero_grad()
         loss.backward()
+        torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
         opt.step()
         sched.step()
         if log and (step % 200 == 0 or step == steps - 1):
-            log(f"train step {step} loss {loss.item():.4f}")
-    return flatten(model.cpu())
+            log(f"train step {step} crop {c} loss {loss.item():.4f}")
+    errs = []
+    model.eval()
+    with torch.no_grad():
+        for f, t in held[:64]:
+            x = torch.from_numpy(f).permute(3, 0, 1, 2)[None].to(device)
+            pr = model(x)[0, 0].cpu().numpy()
+            errs.append(float(np.linalg.norm(pr - t) / (np.linalg.norm(t) + 1e-30)))
+    return flatten(model.cpu()), float(np.mean(errs)) if errs else float("nan")
 
 
 def training_problems(seed=20261210, small=False):
@@ -193,17 +213,34 @@ def training_problems(seed=20261210, small=False):
     return probs
 
 
+def save_samples(path, samples):
+    arrs = {}
+    for i, (f, t) in enumerate(samples):
+        arrs[f"f{i}"] = f
+        arrs[f"t{i}"] = t
+    np.savez(path, **arrs)
+
+
+def load_samples(path):
+    z = np.load(path, allow_pickle=False)
+    n = len(z.files) // 2
+    return [(z[f"f{i}"], z[f"t{i}"]) for i in range(n)]
+
+
 def main(argv=None):
-    """Generate training data with the batched solver and train the correction:
-    python -m doma_amd.unet OUT.npy [--steps N] [--small]."""
+    """Training data and weights for the correction, in two processes (the
+    native library and PyTorch each load their own HIP runtime, and only one of
+    them can own the GPU in a process):
+      python -m doma_amd.unet data SAMPLES.npz [--small] [--sample-cap K]
+      python -m doma_amd.unet train SAMPLES.npz OUT.npy [--steps N]"""
     import argparse
     import sys
     import time
 
-    from .capi import Batch
-
     ap = argparse.ArgumentParser()
-    ap.add_argument("out")
+    ap.add_argument("phase", choices=["data", "train"])
+    ap.add_argument("samples")
+    ap.add_argument("out", nargs="?")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--sample-cap", type=int, default=8)
@@ -213,24 +250,32 @@ def main(argv=None):
         print(m, file=sys.stderr, flush=True)
 
     t0 = time.time()
-    probs = training_problems(small=args.small)
-    # one batch per quadrature (the batch shares one)
-    groups = {}
-    for p in probs:
-        groups.setdefault(p.dirs, []).append(p)
-    samples = []
-    for dirs, ps in groups.items():
-        log(f"data: batch of {len(ps)} problems, {dirs} directions per octant")
-        with Batch(ps) as b:
-            b.autotune()
-            stats, info = b.solve(1e-6, 5000, 1e-6, 60, sample_every=5,
-                                  sample_cap=args.sample_cap)
-            log(f"data: {info['rounds']} rounds, {info['samples']} samples, "
-                f"{info['elapsed_ms'] / 1e3:.1f} s")
-            samples += collect_samples(b, ps)
-    log(f"data: {len(samples)} pairs in {time.time() - t0:.0f} s; training")
-    flat = train(samples, steps=args.steps, log=log)
+    if args.phase == "data":
+        from .capi import Batch
+
+        probs = training_problems(small=args.small)
+        groups = {}
+        for p in probs:  # one batch per quadrature (a batch shares one)
+            groups.setdefault(p.dirs, []).append(p)
+        samples = []
+        for dirs, ps in groups.items():
+            log(f"data: batch of {len(ps)} problems, {dirs} directions per octant")
+            with Batch(ps) as b:
+                b.autotune()
+                stats, info = b.solve(1e-6, 5000, 1e-6, 60, sample_every=5,
+                                      sample_cap=args.sample_cap)
+                ok = sum(1 for s in stats if s["success"])
+                log(f"data: {info['rounds']} rounds, {ok}/{len(ps)} converged, "
+                    f"{info['samples']} samples, {info['elapsed_ms'] / 1e3:.1f} s")
+                samples += collect_samples(b, ps)
+        save_samples(args.samples, samples)
+        log(f"data: {len(samples)} pairs written to {args.samples} in {time.time() - t0:.0f} s")
+        return
+    samples = load_samples(args.samples)
+    log(f"train: {len(samples)} pairs")
+    flat, held_err = train(samples, steps=args.steps, log=log)
     np.save(args.out, flat)
+    log(f"held-out relative L2 error of the predicted correction: {held_err:.3f}")
     log(f"saved {args.out} ({flat.size} parameters) after {time.time() - t0:.0f} s")
 
 

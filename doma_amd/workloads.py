@@ -103,6 +103,18 @@ def high_scatter(n: int = 128, groups: int = 8, np_: int = 8, na: int = 4,
     return p
 
 
+def diffusive(n: int = 128, np_: int = 8, na: int = 4, seed: int = SEED_BASE + 5,
+              c_lo: float = 0.95, c_hi: float = 0.995) -> Problem:
+    """One group, in-group scattering ratio c ~ U[c_lo, c_hi] per block, log-uniform
+    sigma_t in [0.5, 10], sources in 5% of blocks: diffusive media where plain
+    source iteration converges slowly (spectral radius near c)."""
+    spec = MediaSpec(n=n, block=8, groups=1, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0, c_lo=c_lo,
+                     c_hi=c_hi, scatter_rule=capi.SCATTER_DOWN1, source_rule=capi.SOURCE_BLOCKS,
+                     src_lo=0, src_hi=0, source_fraction=0.05, cell_width=1.0, seed=seed)
+    return block_media_problem(n, 1, np_, na, capi.SCHEME_TW_DD, spec,
+                               f"diffusive_{n}c_1g_{8 * np_ * na}d")
+
+
 def batched_uneven(count: int = 512, seed: int = SEED_BASE + 3, np_: int = 4, na: int = 4,
                    sizes=(16, 24, 32, 48, 64), groups=(1, 2, 4)):
     """Independent problems: cube edge and group count drawn per problem."""

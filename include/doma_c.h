@@ -169,6 +169,12 @@ int64_t doma_net_param_count(void);
  * the one before it.  every = 0 disables it; params may then be NULL. */
 int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t count, int32_t every);
 
+/* Accelerated multigroup iteration (no reference counterpart; off by default):
+ * from the second multigroup pass on, each group's source iteration starts from
+ * the previous pass's flux instead of zero (the reference restarts from zero,
+ * TransportSolver3D.cpp:399-400).  2D/3D. */
+int doma_set_warm_start(doma_ctx *ctx, int32_t enable);
+
 /* Stand-alone forward pass (diagnostics and tests): features[V][3] (FP32, V =
  * D*H*W, x fastest) -> out[V]. */
 int doma_net_forward(const float *params, int64_t count, int32_t D, int32_t H, int32_t W,

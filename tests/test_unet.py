@@ -124,7 +124,21 @@ def test_correction_rejects_wrong_inputs():
     with doma_amd.Context(p) as ctx:
         with pytest.raises(doma_amd.DomaError):
             ctx.set_learned_correction(np.zeros(10, np.float32), every=5)
-    p2 = workloads.random_blocks(18, 1, 2, 2)  # 18 is not divisible by 4
+    p2 = workloads.random_blocks(18, 1, 2, 2, block=6)  # 18 is not divisible by 4
     with doma_amd.Context(p2) as ctx:
         with pytest.raises(doma_amd.DomaError):
             ctx.set_learned_correction(np.zeros(unet.param_count(), np.float32), every=5)
+
+
+@pytest.mark.gpu
+def test_warm_start_same_fixed_point():
+    """Warm-started group iterations (accelerated mode) reach the plain fixed
+    point (diamond difference: linear SI) in fewer sweeps."""
+    p = workloads.multigroup_full(16, 4, 2, 2, capi.SCHEME_DD)
+    with doma_amd.Context(p) as ctx:
+        phi0, st0 = ctx.solve_fixed_source(1e-11, 5000, 1e-11, 200)
+        ctx.set_warm_start(True)
+        phi1, st1 = ctx.solve_fixed_source(1e-11, 5000, 1e-11, 200)
+    assert st0["success"] and st1["success"]
+    assert st1["sweeps"] < st0["sweeps"]
+    assert rel_err(phi1, phi0).max() < 1e-8
