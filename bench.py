@@ -128,26 +128,23 @@ def batch_mode(args):
     print(json.dumps(out), flush=True)
 
 
-def learned_mode(args):
+def learned_compare(workload="high_scatter", n=64, groups=8, dd=True, every=5, reference=False,
+                    learned_only=False):
     """Learned-corrected FP32-sweep SI against plain FP64 GPU SI on the same
-    high-scattering problem, both to a 1e-6 residual, and both against plain
-    FP64 SI converged to 1e-9 (no corrections).  With --dd the cells use
+    high-scattering problem, both to a 1e-6 residual, optionally both against
+    plain FP64 SI converged to 1e-9 (no corrections).  With dd the cells use
     diamond difference, whose source iteration is linear and has one fixed point;
     the reference's increment-form TW-DD is not additive (its weights depend on
     each increment, and switch to DD below 1e-8), so its converged flux depends
-    on the iteration path and a corrected path ends at a slightly different one."""
+    on the iteration path and a corrected path ends at a different one."""
     import numpy as np
 
     from doma_amd import workloads
-    from doma_amd.capi import FP32_SWEEP, FP64, Context
+    from doma_amd.capi import FP32_SWEEP, FP64, SCHEME_DD, Context
 
     weights = np.load(ROOT / "doma_amd" / "weights" / "si_correction_unet.npy")
-    if args.workload == "diffusive":
-        p = workloads.diffusive(args.n)
-    else:
-        p = workloads.high_scatter(args.n, args.groups)
-    if args.dd:
-        from doma_amd.capi import SCHEME_DD
+    p = workloads.diffusive(n) if workload == "diffusive" else workloads.high_scatter(n, groups)
+    if dd:
         p.scheme = SCHEME_DD
         p.name += "_dd"
     tol, fs_max, mg_max = 1e-6, 50000, 500
@@ -157,7 +154,7 @@ def learned_mode(args):
         ctx = Context(p, precision=precision)
         ctx.autotune(0)
         if learned:
-            ctx.set_learned_correction(weights, every=args.every)
+            ctx.set_learned_correction(weights, every=every)
         if warm:
             ctx.set_warm_start(True)
         t0 = time.perf_counter()
@@ -168,12 +165,17 @@ def learned_mode(args):
             f"corrections {st['corrections_accepted']}+/{st['corrections_rejected']}-")
         return phi, st, wall
 
+    if learned_only:  # profiling: the accelerated solve alone
+        _, st_w, t_w = run(FP32_SWEEP, True, tol, "learned FP32 + warm start", warm=p.groups > 1)
+        return {"workload": p.name, "wall_s": round(t_w, 3), "sweeps": st_w["sweeps"],
+                "mg_iterations": st_w["mg_iterations"],
+                "correction_s": round(st_w["correction_ms"] / 1e3, 3)}
     phi_p, st_p, t_p = run(FP64, False, tol, "plain FP64")
     phi_l, st_l, t_l = run(FP32_SWEEP, True, tol, "learned FP32")
     phi_w, st_w, t_w = (None, None, None)
     if p.groups > 1:
         phi_w, st_w, t_w = run(FP32_SWEEP, True, tol, "learned FP32 + warm start", warm=True)
-    out = {"workload": p.name, "tolerance": tol, "every": args.every,
+    out = {"workload": p.name, "tolerance": tol, "every": every,
            "plain_fp64": {"wall_s": round(t_p, 3), "sweeps": st_p["sweeps"],
                           "mg_iterations": st_p["mg_iterations"], "success": st_p["success"]},
            "learned_fp32": {"wall_s": round(t_l, 3), "sweeps": st_l["sweeps"],
@@ -189,8 +191,9 @@ def learned_mode(args):
             "corrections_accepted": st_w["corrections_accepted"],
             "corrections_rejected": st_w["corrections_rejected"]}
         out["speedup_warm"] = round(t_p / t_w, 3)
-    if not args.no_reference:
+    if reference:
         phi_r, st_r, t_r = run(FP64, False, 1e-9, "reference: plain FP64 to 1e-9")
+
         def errs(phi):
             d = np.abs(phi - phi_r)
             return {"max_rel": [float(x) for x in d.max(axis=1) / np.abs(phi_r).max(axis=1)],
@@ -202,7 +205,18 @@ def learned_mode(args):
         out["learned_fp32"]["error"] = errs(phi_l)
         if st_w is not None:
             out["learned_fp32_warm"]["error"] = errs(phi_w)
-    print(json.dumps(out), flush=True)
+    else:
+        # the accelerated result against plain SI's (both stopped at 1e-6)
+        ref = phi_w if phi_w is not None else phi_l
+        out["accelerated_vs_plain_max_rel"] = [
+            float(x) for x in np.abs(ref - phi_p).max(axis=1) / np.abs(phi_p).max(axis=1)]
+    return out
+
+
+def learned_mode(args):
+    print(json.dumps(learned_compare(args.workload if args.workload in ("diffusive", "high_scatter")
+                                     else "high_scatter", args.n, args.groups, args.dd, args.every,
+                                     not args.no_reference, args.learned_only)), flush=True)
 
 
 def copy_calibration(nbytes=256 << 20, reps=3):
@@ -226,6 +240,7 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain solve")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
     ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned"],
                     default="solve")
@@ -233,6 +248,7 @@ def main():
     ap.add_argument("--no-reference", action="store_true")
     ap.add_argument("--dd", action="store_true", help="diamond difference (learned mode)")
     ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
+    ap.add_argument("--learned-only", action="store_true", help="learned mode: accelerated run only")
     ap.add_argument("--count", type=int, default=512, help="problems (batch mode)")
     ap.add_argument("--sample-cap", type=int, default=8, help="snapshots per problem and group")
     ap.add_argument("--workload", choices=["random_blocks", "multigroup_full", "high_scatter",
@@ -306,8 +322,22 @@ def main():
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                 "traffic": traffic,
-                "kernel": "brick_sweep_kernel", "avg_launch_us": round(k_ms * 1e3 / launches, 2),
+                "kernel": "brick_flow_kernel" if tuning.get("schedule") == 1 else "brick_sweep_kernel",
+                "avg_launch_us": round(k_ms * 1e3 / launches, 2),
                 "launches_per_sweep": launches, "flops_per_unit": FLOPS_PER_UNIT[p.scheme]}
+    ctx.close()
+    learned = None
+    if not args.no_learned:
+        lc = learned_compare("high_scatter", 64, 8, dd=True, every=5)
+        learned = {"workload": lc["workload"], "tolerance": lc["tolerance"],
+                   "plain_fp64_s": lc["plain_fp64"]["wall_s"],
+                   "learned_fp32_s": lc["learned_fp32"]["wall_s"],
+                   "learned_fp32_warm_start_s": lc["learned_fp32_warm"]["wall_s"],
+                   "speedup_learned": lc["speedup"], "speedup_learned_warm_start": lc["speedup_warm"],
+                   "sweeps": {"plain": lc["plain_fp64"]["sweeps"],
+                              "learned": lc["learned_fp32"]["sweeps"],
+                              "learned_warm_start": lc["learned_fp32_warm"]["sweeps"]},
+                   "max_rel_vs_plain": lc["accelerated_vs_plain_max_rel"]}
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -334,11 +364,10 @@ def main():
         "sweeps_per_solve": sweeps // args.steps,
         "sweep_kernel_grind_ns": round(k_ms * 1e6 / units_sweep, 5),
         "sweep_wall_grind_ns": round(w_ms * 1e6 / units_sweep, 5),
-        "learned_vs_plain": None,
+        "learned_vs_plain": learned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 
