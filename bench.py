@@ -213,6 +213,48 @@ def learned_compare(workload="high_scatter", n=64, groups=8, dd=True, every=5, r
     return out
 
 
+def precision_mode(args):
+    """FP32-sweep mode against FP64 on the 8-group full-scatter problem (FP32
+    angular flux and cell solve; FP64 scalar-flux accumulation, group source and
+    residuals): time to a 1e-6 residual and the max / L2 relative scalar-flux
+    error of FP32 per group, with FP64 as the reference."""
+    import numpy as np
+
+    from doma_amd import workloads
+    from doma_amd.capi import FP32_SWEEP, FP64, Context
+
+    p = workloads.multigroup_full(args.n, 8, 8, 4)
+    log(f"bench: {p.name} N={p.N} groups={p.groups} directions={p.noct * p.dirs}")
+    res = {}
+    for label, prec in (("fp64", FP64), ("fp32_sweep", FP32_SWEEP)):
+        ctx = Context(p, precision=prec)
+        tuning, _ = ctx.autotune(0)
+        t0 = time.perf_counter()
+        phi, st = ctx.solve_fixed_source(1e-6, 5000, 1e-6, 500)
+        wall = time.perf_counter() - t0
+        ctx.close()
+        log(f"bench: {label}: {wall:.2f} s, {st['sweeps']} sweeps, MG {st['mg_iterations']}")
+        res[label] = (phi, st, wall, tuning)
+    ref, got = res["fp64"][0], res["fp32_sweep"][0]
+    d = np.abs(got - ref)
+    out = {"workload": p.name, "tolerance": 1e-6,
+           "fp64": {"wall_s": round(res["fp64"][2], 3), "sweeps": res["fp64"][1]["sweeps"],
+                    "mg_iterations": res["fp64"][1]["mg_iterations"],
+                    "sweep_ms": round(res["fp64"][1]["sweep_ms"] / res["fp64"][1]["sweeps"], 4),
+                    "tuning": res["fp64"][3]},
+           "fp32_sweep": {"wall_s": round(res["fp32_sweep"][2], 3),
+                          "sweeps": res["fp32_sweep"][1]["sweeps"],
+                          "mg_iterations": res["fp32_sweep"][1]["mg_iterations"],
+                          "sweep_ms": round(res["fp32_sweep"][1]["sweep_ms"] /
+                                            res["fp32_sweep"][1]["sweeps"], 4),
+                          "tuning": res["fp32_sweep"][3]},
+           "fp32_error_vs_fp64": {
+               "max_rel": [float(x) for x in d.max(axis=1) / np.abs(ref).max(axis=1)],
+               "l2_rel": [float(x) for x in np.linalg.norm(got - ref, axis=1) /
+                          np.linalg.norm(ref, axis=1)]}}
+    print(json.dumps(out), flush=True)
+
+
 def learned_mode(args):
     print(json.dumps(learned_compare(args.workload if args.workload in ("diffusive", "high_scatter")
                                      else "high_scatter", args.n, args.groups, args.dd, args.every,
@@ -242,7 +284,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain solve")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
-    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned"],
+    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned", "precision"],
                     default="solve")
     ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
     ap.add_argument("--no-reference", action="store_true")
@@ -270,6 +312,8 @@ def main():
         return batch_mode(args)
     if args.mode == "learned":
         return learned_mode(args)
+    if args.mode == "precision":
+        return precision_mode(args)
 
     from doma_amd import workloads
     from doma_amd.capi import FP64, Context
