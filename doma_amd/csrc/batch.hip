@@ -153,9 +153,7 @@ configure(doma_batch * b)
   t.schedule = 1;
   t.exact_arith = 0;
   const doma_ctx * c0 = b->ctx[0];
-  const int nchunks = (c0->dirs + 63) / 64;
-  const int chunk = (c0->dirs + nchunks - 1) / nchunks;
-  const int shape = select_shape(3, 8, chunk, minext, t);
+  const int shape = select_shape(3, 8, c0->dirs, minext, t);
   if (shape < 0)
     return fail(DOMA_ERR_INVALID, "no 3D sweep kernel shape matches the batch tuning");
   t.shape = shape + 1;
@@ -558,8 +556,6 @@ doma_batch_autotune(doma_batch * b, doma_tuning * best, double * best_ms)
   if (rc)
     return rc;
   const doma_ctx * c0 = b->ctx[0];
-  const int nchunks = (c0->dirs + 63) / 64;
-  const int chunk = (c0->dirs + nchunks - 1) / nchunks;
   int minext = 1 << 30;
   for (const auto * c : b->ctx)
     minext = std::min(minext, std::min(c->nx, std::min(c->ny, c->nz)));
@@ -571,7 +567,7 @@ doma_batch_autotune(doma_batch * b, doma_tuning * best, double * best_ms)
     doma_tuning t{};
     t.shape = s + 1;
     t.schedule = 1;
-    if (select_shape(3, 8, chunk, minext, t) != s)
+    if (select_shape(3, 8, c0->dirs, minext, t) != s)
       continue;
     b->tuning = t;
     if (configure(b) != DOMA_OK)

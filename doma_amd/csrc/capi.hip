@@ -147,41 +147,57 @@ reset_after_sweep_error(doma_ctx * c)
   return fail(DOMA_ERR_DEVICE, "sweep dataflow wait expired: an upwind brick never finished");
 }
 
-// Kernel shape for a problem: shapes of the right dimension whose direction
-// width covers the chunk, whose octant group divides the octant count and that
-// agree with every non-zero tuning hint.  Without hints the default prefers the
-// narrowest direction width, then 8^3 bricks on large grids (4^3 otherwise; 16^2
-// / 8^2 tiles in 2D), then one direction per lane and one octant per group
-// (measured fastest on MI355X: the sweep is latency bound, and those shapes
-// give the shortest per-brick critical path at the highest occupancy).
+// Direction sets a shape implies: at most min(64, DL * NDL) directions per set,
+// spread evenly.
+void
+shape_chunking(int shape, int dirs, int * chunk, int * nchunks)
+{
+  ShapeInfo si{};
+  shape_info(shape, &si);
+  const int w = std::max(1, std::min(64, si.dl * si.ndl));
+  *nchunks = (dirs + w - 1) / w;
+  *chunk = (dirs + *nchunks - 1) / *nchunks;
+}
+
+// Kernel shape for a problem: shapes of the right dimension whose octant group
+// divides the octant count and that agree with every non-zero tuning hint.
+// Without hints the default prefers the fewest direction sets (one per 64
+// directions) and the narrowest direction width that covers them, then 8^3
+// bricks on large grids (4^3 otherwise; 16^2 / 8^2 tiles in 2D), then one
+// direction per lane and one octant per group (measured fastest on MI355X: the
+// sweep is latency bound, and those shapes give the shortest per-brick critical
+// path at the highest occupancy).  Shapes that split the directions further are
+// left to the autotuner.
 int
-select_shape(int dims, int noct, int chunk, int minext, const doma_tuning & t)
+select_shape(int dims, int noct, int dirs, int minext, const doma_tuning & t)
 {
   if (t.shape > 0)
   {
     ShapeInfo si{};
-    if (!shape_info(t.shape - 1, &si) || si.dims != dims || si.dl * si.ndl < chunk ||
-        noct % si.ko != 0)
+    if (!shape_info(t.shape - 1, &si) || si.dims != dims || noct % si.ko != 0)
       return -1;
     return t.shape - 1;
   }
   const int pref_brick = dims == 3 ? (minext >= 96 ? 8 : 4) : (minext >= 128 ? 16 : 8);
+  const int min_sets = (dirs + 63) / 64;
   int best = -1;
   long best_score = 0;
   for (int s = 0; s < num_shapes(); ++s)
   {
     ShapeInfo si{};
     shape_info(s, &si);
-    const int dw = si.dl * si.ndl;
-    if (si.dims != dims || dw < chunk || noct % si.ko != 0)
+    if (si.dims != dims || noct % si.ko != 0)
       continue;
     if ((t.brick && si.brick != t.brick) || (t.threads && si.threads != t.threads) ||
         (t.octants && si.ko != t.octants) || (t.dirs_per_lane && si.ndl != t.dirs_per_lane))
       continue;
+    int chunk = 0, nchunks = 0;
+    shape_chunking(s, dirs, &chunk, &nchunks);
+    const int dw = si.dl * si.ndl;
     const int pref_threads = si.brick == 8 ? 512 : 256;
-    const long score = (long)(dw - chunk) * 1000 + (si.brick != pref_brick ? 100 : 0) +
-                       (si.ndl == 1 ? 0 : 20) + (si.ko - 1) * 4 +
-                       (si.threads != pref_threads ? 1 : 0);
+    const long score = (long)(nchunks - min_sets) * 100000 + (long)(dw - chunk) * 1000 +
+                       (si.brick != pref_brick ? 100 : 0) + (si.ndl == 1 ? 0 : 20) +
+                       (si.ko - 1) * 4 + (si.threads != pref_threads ? 1 : 0);
     if (best < 0 || score < best_score)
     {
       best = s;
@@ -204,13 +220,12 @@ configure_sweep(doma_ctx * c)
     return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
   }
   const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
-  c->nchunks = (c->dirs + 63) / 64;
-  c->chunk = (c->dirs + c->nchunks - 1) / c->nchunks;
-  const int shape = select_shape(c->dims, c->noct, c->chunk, minext, c->tuning);
+  const int shape = select_shape(c->dims, c->noct, c->dirs, minext, c->tuning);
   ShapeInfo si{};
   if (shape < 0 || !shape_info(shape, &si))
     return fail(DOMA_ERR_INVALID, "no sweep kernel shape matches the tuning for " +
-                                      std::to_string(c->chunk) + " directions per chunk");
+                                      std::to_string(c->dirs) + " directions per octant");
+  shape_chunking(shape, c->dirs, &c->chunk, &c->nchunks);
   const int B = si.brick;
   c->shape = shape;
   c->brick = B;
@@ -1140,7 +1155,7 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
         doma_tuning t{};
         t.shape = s + 1;
         t.schedule = sch;
-        if (select_shape(c->dims, c->noct, c->chunk, minext, t) == s)
+        if (select_shape(c->dims, c->noct, c->dirs, minext, t) == s)
           cands.push_back(t);
       }
   }

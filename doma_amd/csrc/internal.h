@@ -258,7 +258,8 @@ struct doma_ctx
 namespace doma
 {
 int create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out);
-int select_shape(int dims, int noct, int chunk, int minext, const doma_tuning & t);
+int select_shape(int dims, int noct, int dirs, int minext, const doma_tuning & t);
+void shape_chunking(int shape, int dirs, int * chunk, int * nchunks);
 int configure_sweep(doma_ctx * c);
 int run_sweep(doma_ctx * c, int g);
 BrickArgs brick_args(doma_ctx * c, const double * q);

@@ -54,7 +54,9 @@ struct Shape
 };
 
 // Shape table (index = shape id).  Chosen for direction chunks of <= 16, <= 32
-// and <= 64 per octant, small and large bricks, in 3D and 2D.
+// and <= 64 per octant, small and large bricks, in 3D and 2D.  A shape whose
+// direction width DL * NDL is below the directions per octant splits them into
+// several direction sets (more independent work-groups per brick diagonal).
 constexpr Shape kShapes[] = {
     {3, 4, 8, 2, 4, 128},   // 0: <=16 dirs, 4^3 bricks, 4 octants
     {3, 4, 8, 2, 8, 256},   // 1: <=16 dirs, 4^3 bricks, 8 octants
@@ -77,6 +79,11 @@ constexpr Shape kShapes[] = {
     {3, 8, 64, 1, 1, 512},  // 18: <=64 dirs, 8^3 bricks
     {2, 8, 16, 1, 1, 256},  // 19: 2D <=16 dirs
     {2, 16, 32, 1, 1, 512}, // 20: 2D <=32 dirs, 16^2 tiles
+    {3, 8, 8, 1, 1, 512},   // 21: 8-direction sets, 8^3 bricks, 64 cells per pass
+    {3, 8, 16, 1, 1, 1024}, // 22: <=16 dirs, 8^3 bricks, 64 cells per pass
+    {3, 8, 8, 1, 1, 256},   // 23: 8-direction sets, 8^3 bricks, 32 cells per pass
+    {3, 4, 8, 1, 1, 128},   // 24: 8-direction sets, 4^3 bricks
+    {3, 8, 4, 1, 1, 256},   // 25: 4-direction sets, 8^3 bricks, 64 cells per pass
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
@@ -102,16 +109,17 @@ constexpr int DPP_QP_XOR2 = 0x4E;
 constexpr int DPP_HALF_MIRROR = 0x141;
 constexpr int DPP_MIRROR = 0x140;
 
-// Fixed-order sum over groups of DL (8..64) consecutive lanes; every lane of a
+// Fixed-order sum over groups of DL (4..64) consecutive lanes; every lane of a
 // group ends with the same bitwise result (each step adds a commutative pair).
 template <int DL>
 __device__ __forceinline__ double
 group_sum(double v)
 {
-  static_assert(DL >= 8 && DL <= 64, "direction lanes");
+  static_assert(DL >= 4 && DL <= 64, "direction lanes");
   v += dpp_mov<DPP_QP_XOR1>(v);
   v += dpp_mov<DPP_QP_XOR2>(v);
-  v += dpp_mov<DPP_HALF_MIRROR>(v);
+  if constexpr (DL >= 8)
+    v += dpp_mov<DPP_HALF_MIRROR>(v);
   if constexpr (DL >= 16)
     v += dpp_mov<DPP_MIRROR>(v);
   if constexpr (DL >= 32)

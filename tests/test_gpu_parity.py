@@ -85,6 +85,14 @@ SWEEP_SHAPES = [
     (18, 3, 9, 10, 11, 8, 8, capi.SCHEME_TW_DD),
     (19, 2, 21, 18, 1, 4, 4, capi.SCHEME_TW_DD),
     (20, 2, 37, 29, 1, 8, 4, capi.SCHEME_DD),
+    # shapes narrower than the directions per octant: several direction sets
+    (21, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
+    (22, 3, 17, 16, 19, 4, 4, capi.SCHEME_DD),
+    (23, 3, 13, 9, 11, 8, 4, capi.SCHEME_TW_DD),
+    (24, 3, 10, 9, 7, 4, 4, capi.SCHEME_TW_DD),
+    (25, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
+    (25, 3, 9, 10, 11, 2, 3, capi.SCHEME_DD),
+    (16, 3, 12, 10, 9, 8, 4, capi.SCHEME_TW_DD),
 ]
 
 
