@@ -121,8 +121,10 @@ shape_lds_bytes(int shape, int precision)
   const int fs = si.dims == 3 ? si.brick * si.brick : si.brick;
   const int nc = si.dims == 3 ? si.brick * si.brick * si.brick : si.brick * si.brick;
   const int nax = si.dims == 3 ? 3 : 2;
-  return (rs * fs * si.dl * si.ndl * nax * si.ko + 15) / 16 * 16 + 20 * (size_t)nc * si.ko +
-         24 * (size_t)si.brick * si.ko + 2 * (size_t)nc + 4 * (size_t)(3 * si.brick) + 32;
+  const int nplanes = si.dims == 3 ? 3 * si.brick - 2 : 2 * si.brick - 1;
+  // faces, cell records (48 B), plane table, plane offsets, task slot
+  return (rs * fs * si.dl * si.ndl * nax * si.ko + 15) / 16 * 16 + 48 * (size_t)nc * si.ko +
+         4 * (size_t)nc + 4 * (size_t)(nplanes + 1) + 16;
 }
 
 int

@@ -95,10 +95,12 @@ template <int CTRL>
 __device__ __forceinline__ double
 dpp_mov(double v)
 {
+  // Every lane of the permutations used here reads a valid lane of its own row,
+  // so no lane takes the bound-control value and no old value is needed.
   const int lo = __double2loint(v);
   const int hi = __double2hiint(v);
-  const int rlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+  const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
   return __hiloint2double(rhi, rlo);
 }
 
@@ -133,6 +135,19 @@ group_sum(double v)
 // LDS layout                                                                 //
 // ------------------------------------------------------------------------- //
 
+// Per-cell record staged in LDS for a brick: source and cross-section terms,
+// the geometry factors the cell solve needs, and the global cell index (-1 for
+// a cell of a partial brick outside the grid).
+//   fast TW-DD 3D: v0 = q V, v1 = sigma V, g = (ly lz, lx lz, lx ly)
+//   fast TW-DD 2D: v0 = q V, v1 = sigma V, g = (ly, lx, 1)
+//   fast DD:       v0 = q,   v1 = sigma,   g = (1/lx, 1/ly, 1/lz)
+//   exact:         v0 = q,   v1 = sigma,   g = (lx, ly, lz)
+struct alignas(16) CellRec
+{
+  double v0, v1, g0, g1, g2;
+  int c, pad;
+};
+
 template <typename R, int DIM, int B, int DL, int NDL, int KO>
 struct Layout
 {
@@ -143,24 +158,33 @@ struct Layout
   static constexpr int NAX = DIM == 3 ? 3 : 2;
   static constexpr size_t FACE = sizeof(R) * FS * DW;      // one axis, one octant
   static constexpr size_t FACES_OFF = 0;
-  static constexpr size_t CELL_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;  // double2 [KO][NC]
-  static constexpr size_t GIDX_OFF = CELL_OFF + 16 * NC * KO;             // int [KO][NC]
-  static constexpr size_t GEO_OFF = GIDX_OFF + 4 * NC * KO;               // double [KO][3][B]
-  static constexpr size_t PT_OFF = GEO_OFF + 8 * 3 * B * KO;              // uint16 [NC]
-  static constexpr size_t PO_OFF = (PT_OFF + 2 * NC + 3) / 4 * 4;         // int [NPLANES+1]
+  static constexpr size_t REC_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;   // CellRec [KO][NC]
+  static constexpr size_t PT_OFF = REC_OFF + sizeof(CellRec) * NC * KO;  // uint32 [NC]
+  static constexpr size_t PO_OFF = PT_OFF + 4 * NC;                        // int [NPLANES+1]
   static constexpr size_t TASK_OFF = PO_OFF + 4 * (NPLANES + 1);
   static constexpr size_t BYTES = TASK_OFF + 16;
 };
 
+// Plane table: per position of the plane-ordered cell list, the brick-local cell
+// ce and its x-, y- and z-face slots packed as ce | sx << 9 | sy << 15 | sz << 21
+// (3D: sx = lj B + lk, sy = li B + lk, sz = li B + lj; 2D: sx = lj, sy = li).
 template <typename R, int DIM, int B, int DL, int NDL, int KO, int NT>
 __device__ __forceinline__ void
 load_plane_table(const BrickArgs & a, unsigned char * smem)
 {
   using L = Layout<R, DIM, B, DL, NDL, KO>;
-  uint16_t * pt = reinterpret_cast<uint16_t *>(smem + L::PT_OFF);
+  static_assert(L::NC <= 512 && L::FS <= 64, "plane table packing");
+  uint32_t * pt = reinterpret_cast<uint32_t *>(smem + L::PT_OFF);
   int32_t * po = reinterpret_cast<int32_t *>(smem + L::PO_OFF);
   for (int e = threadIdx.x; e < L::NC; e += NT)
-    pt[e] = static_cast<uint16_t>(a.plane_cells[e]);
+  {
+    const uint32_t ce = a.plane_cells[e];
+    const uint32_t li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
+    const uint32_t sx = DIM == 3 ? lj * B + lk : lj;
+    const uint32_t sy = DIM == 3 ? li * B + lk : li;
+    const uint32_t sz = DIM == 3 ? li * B + lj : 0;
+    pt[e] = ce | (sx << 9) | (sy << 15) | (sz << 21);
+  }
   for (int e = threadIdx.x; e <= L::NPLANES; e += NT)
     po[e] = a.plane_off[e];
 }
@@ -216,10 +240,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   static_assert(CPP % KO == 0, "cell slots must split evenly over octants");
 
   R * faces = reinterpret_cast<R *>(smem + L::FACES_OFF);  // [KO][NAX][FS][DW]
-  double2 * cd = reinterpret_cast<double2 *>(smem + L::CELL_OFF);
-  int * gidx = reinterpret_cast<int *>(smem + L::GIDX_OFF);
-  double * geo = reinterpret_cast<double *>(smem + L::GEO_OFF);
-  const uint16_t * pt = reinterpret_cast<const uint16_t *>(smem + L::PT_OFF);
+  CellRec * rec = reinterpret_cast<CellRec *>(smem + L::REC_OFF);
+  const uint32_t * pt = reinterpret_cast<const uint32_t *>(smem + L::PT_OFF);
   const int32_t * po = reinterpret_cast<const int32_t *>(smem + L::PO_OFF);
 
   const int tid = threadIdx.x;
@@ -280,7 +302,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
                                          (j0 + v)) * DW + l);
     faces[e] = val;
   }
-  // Cell data of every octant, brick-local order (lk*B + lj)*B + li.
+  // Cell records of every octant, brick-local order (lk*B + lj)*B + li.
   for (int e = tid; e < KO * NC; e += NT)
   {
     const int k_o = e / NC, ce = e - k_o * NC;
@@ -288,66 +310,69 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
     orient<DIM>(og * KO + k_o, nx, ny, nz);
     const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
     const int oi = i0 + li, oj = j0 + lj, ok = k0 + lk;
-    double2 v = make_double2(0.0, 1.0);
-    int c = -1;
+    CellRec r;
+    r.v0 = 0.0;
+    r.v1 = 1.0;
+    r.g0 = r.g1 = r.g2 = 1.0;
+    r.c = -1;
+    r.pad = 0;
     if (oi < a.nx && oj < a.ny && ok < a.nz)
     {
       const int i = nx ? a.nx - 1 - oi : oi;
       const int j = ny ? a.ny - 1 - oj : oj;
       const int k = nz ? a.nz - 1 - ok : ok;
-      c = (k * a.ny + j) * a.nx + i;
+      const int c = (k * a.ny + j) * a.nx + i;
       const double q = a.q[c], sg = a.sig[c];
-      if (EXACT || SCHEME == DOMA_SCHEME_DD)
-        v = make_double2(q, sg);
+      const double lx = a.lx[i], ly = a.ly[j], lz = DIM == 3 ? a.lz[k] : 1.0;
+      r.c = c;
+      if (EXACT)
+      {
+        r.v0 = q;
+        r.v1 = sg;
+        r.g0 = lx;
+        r.g1 = ly;
+        r.g2 = lz;
+      }
+      else if (SCHEME == DOMA_SCHEME_DD)
+      {
+        r.v0 = q;
+        r.v1 = sg;
+        r.g0 = 1.0 / lx;
+        r.g1 = 1.0 / ly;
+        r.g2 = 1.0 / lz;
+      }
       else
       {
         const double V = DIM == 3 ? a.lx[i] * a.ly[j] * a.lz[k] : a.lx[i] * a.ly[j];
-        v = make_double2(q * V, sg * V);
+        r.v0 = q * V;
+        r.v1 = sg * V;
+        r.g0 = DIM == 3 ? ly * lz : ly;
+        r.g1 = DIM == 3 ? lx * lz : lx;
+        r.g2 = DIM == 3 ? lx * ly : 1.0;
       }
     }
-    cd[e] = v;
-    gidx[e] = c;
-  }
-  for (int e = tid; e < KO * 3 * B; e += NT)
-  {
-    const int k_o = e / (3 * B), r = e - k_o * 3 * B;
-    const int ax = r / B, l = r - ax * B;
-    bool nx, ny, nz;
-    orient<DIM>(og * KO + k_o, nx, ny, nz);
-    double v = 1.0;
-    if (ax == 0 && i0 + l < a.nx)
-      v = a.lx[nx ? a.nx - 1 - (i0 + l) : i0 + l];
-    else if (ax == 1 && j0 + l < a.ny)
-      v = a.ly[ny ? a.ny - 1 - (j0 + l) : j0 + l];
-    else if (ax == 2 && DIM == 3 && k0 + l < a.nz)
-      v = a.lz[nz ? a.nz - 1 - (k0 + l) : k0 + l];
-    geo[e] = v;
+    rec[e] = r;
   }
   __syncthreads();
 
-  R * fxo = faces + (size_t)ko * NAX * FS * DW;  // this lane's octant
+  R * fxo = faces + (size_t)ko * NAX * FS * DW + dd * NDL;  // this lane's octant and directions
   R * fyo = fxo + FS * DW;
   R * fzo = fyo + FS * DW;
-  const double2 * cdo = cd + ko * NC;
-  const int * gio = gidx + ko * NC;
-  const double * gxo = geo + ko * 3 * B;
+  const CellRec * rco = rec + ko * NC;
 
   for (int p = 0; p < L::NPLANES; ++p)
   {
     const int pe = po[p + 1];
     for (int pos = po[p] + cpo; pos < pe; pos += CPO)
     {
-      const int ce = pt[pos];
-      const int c = gio[ce];
-      if (c < 0)
-        continue;  // outside a partial brick: the whole lane group skips
-      const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
-      const double2 cv = cdo[ce];
-      const double lxd = gxo[li], lyd = gxo[B + lj];
-      const double lzd = DIM == 3 ? gxo[2 * B + lk] : 1.0;
-      R * xf = fxo + (DIM == 3 ? (lj * B + lk) : lj) * DW + dd * NDL;
-      R * yf = fyo + (DIM == 3 ? (li * B + lk) : li) * DW + dd * NDL;
-      R * zf = fzo + (li * B + lj) * DW + dd * NDL;
+      // Cells of a partial brick outside the grid are solved too (on finite
+      // dummy data): they only feed other outside cells, and their scalar flux
+      // is not stored.
+      const uint32_t info = pt[pos];
+      const CellRec r = rco[info & 511u];
+      R * xf = fxo + ((info >> 9) & 63u) * DW;
+      R * yf = fyo + ((info >> 15) & 63u) * DW;
+      R * zf = fzo + ((info >> 21) & 63u) * DW;
       double s = 0.0;
 #pragma unroll
       for (int m = 0; m < NDL; ++m)
@@ -360,21 +385,20 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
           if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
           {
             if constexpr (EXACT)
-              psi = twdd3d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), R(lzd), amu[m], aeta[m],
+              psi = twdd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
                                     axi[m], x, y, z);
             else
-              psi = twdd3d_fast<R>(R(cv.x), R(cv.y), amu[m] * R(lyd * lzd),
-                                   aeta[m] * R(lxd * lzd), axi[m] * R(lxd * lyd), x, y, z);
+              psi = twdd3d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1),
+                                   axi[m] * R(r.g2), x, y, z);
           }
           else
           {
             if constexpr (EXACT)
-              psi = dd3d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), R(lzd), amu[m], aeta[m],
+              psi = dd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
                                   axi[m], x, y, z);
             else
-              psi = dd3d_fast<R>(R(cv.x), R(cv.y), R(2) * amu[m] * R(1.0 / lxd),
-                                 R(2) * aeta[m] * R(1.0 / lyd), R(2) * axi[m] * R(1.0 / lzd), x, y,
-                                 z);
+              psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
+                                 R(2) * aeta[m] * R(r.g1), R(2) * axi[m] * R(r.g2), x, y, z);
           }
         }
         else
@@ -382,17 +406,19 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
           if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
           {
             if constexpr (EXACT)
-              psi = twdd2d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), amu[m], aeta[m], axi[m], x, y);
+              psi = twdd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], axi[m], x,
+                                    y);
             else
-              psi = twdd2d_fast<R>(R(cv.x), R(cv.y), amu[m] * R(lyd), aeta[m] * R(lxd), axi[m], x, y);
+              psi = twdd2d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1), axi[m], x,
+                                   y);
           }
           else
           {
             if constexpr (EXACT)
-              psi = dd2d_exact<R>(R(cv.x), R(cv.y), R(lxd), R(lyd), amu[m], aeta[m], x, y);
+              psi = dd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], x, y);
             else
-              psi = dd3d_fast<R>(R(cv.x), R(cv.y), R(2) * amu[m] * R(1.0 / lxd),
-                                 R(2) * aeta[m] * R(1.0 / lyd), R(0), x, y, z);
+              psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
+                                 R(2) * aeta[m] * R(r.g1), R(0), x, y, z);
           }
         }
         xf[m] = x;
@@ -402,8 +428,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
         s += w[m] * static_cast<double>(psi);
       }
       s = group_sum<DL>(s);
-      if (dd == 0)
-        a.part[(size_t)ds * a.N + c] = s;
+      if (dd == 0 && r.c >= 0)
+        a.part[(size_t)ds * a.N + r.c] = s;
     }
     __syncthreads();
   }
