@@ -98,6 +98,33 @@ inv_weight(R n, R d)
   return r;
 }
 
+// The three inverse weights of one cell with a single reciprocal (FP64):
+// 1/Da = Db Dc / (Da Db Dc) with Da = da - na etc. (batch inversion).  A weight
+// that is clamped or takes the 0.5 default contributes D = 1, so no zero or
+// non-finite factor can reach the shared reciprocal; an unclamped D lies in
+// (d/2, d] and is well conditioned.  Within a few ulp of three separate divides.
+__device__ __forceinline__ void
+inv_weights3(double na, double da, bool va, double nb, double db, bool vb, double nc, double dc,
+             bool vc, double & ia, double & ib, double & ic)
+{
+  const bool ua = va && na >= 0.0 && na < 0.5 * da;
+  const bool ub = vb && nb >= 0.0 && nb < 0.5 * db;
+  const bool uc = vc && nc >= 0.0 && nc < 0.5 * dc;
+  const double Da = ua ? da - na : 1.0;
+  const double Db = ub ? db - nb : 1.0;
+  const double Dc = uc ? dc - nc : 1.0;
+  const double Dbc = Db * Dc, P = Da * Dbc;
+  double r = __builtin_amdgcn_rcp(P);
+  double e = fma(-P, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-P, r, 1.0);
+  r = fma(r, e, r);
+  const double ra = r * Dbc, rb = r * (Da * Dc), rc = r * (Da * Db);
+  ia = ua ? da * ra : (va && na < 0.0 ? 1.0 : 2.0);
+  ib = ub ? db * rb : (vb && nb < 0.0 ? 1.0 : 2.0);
+  ic = uc ? dc * rc : (vc && nc < 0.0 ? 1.0 : 2.0);
+}
+
 // Straight-line TWDD3D: the three weights are always formed and the reference's
 // "upwind flux > 1e-8" test selects between them and the 0.5 default, so a lane
 // can interleave the solves of its directions.
@@ -109,12 +136,23 @@ twdd3d_fast(R qV, R sV, R mA, R eB, R xC, R & x, R & y, R & z)
   const R qVs = qV * th;
   const R mAx = mA * x, eBy = eB * y, xCz = xC * z;
   const R two_s = sV + R(2) * (mA + eB + xC);
-  R ia = inv_weight(qVs + (eBy + xCz) * th + mAx, (two_s - R(2) * mA) * x);
-  R ib = inv_weight(qVs + (mAx + xCz) * th + eBy, (two_s - R(2) * eB) * y);
-  R ic = inv_weight(qVs + (mAx + eBy) * th + xCz, (two_s - R(2) * xC) * z);
-  ia = x > R(1e-8) ? ia : R(2);
-  ib = y > R(1e-8) ? ib : R(2);
-  ic = z > R(1e-8) ? ic : R(2);
+  const R na = qVs + (eBy + xCz) * th + mAx, da = (two_s - R(2) * mA) * x;
+  const R nb = qVs + (mAx + xCz) * th + eBy, db = (two_s - R(2) * eB) * y;
+  const R nc = qVs + (mAx + eBy) * th + xCz, dc = (two_s - R(2) * xC) * z;
+  R ia, ib, ic;
+  if constexpr (sizeof(R) == 8)
+  {
+    inv_weights3(na, da, x > R(1e-8), nb, db, y > R(1e-8), nc, dc, z > R(1e-8), ia, ib, ic);
+  }
+  else
+  {
+    ia = inv_weight(na, da);
+    ib = inv_weight(nb, db);
+    ic = inv_weight(nc, dc);
+    ia = x > R(1e-8) ? ia : R(2);
+    ib = y > R(1e-8) ? ib : R(2);
+    ic = z > R(1e-8) ? ic : R(2);
+  }
   const R wa = mA * ia, wb = eB * ib, wc = xC * ic;
   const R psi = fast_div(qV + wa * x + wb * y + wc * z, sV + wa + wb + wc);
   x = ia * psi - (ia - R(1)) * x;
