@@ -84,6 +84,9 @@ constexpr Shape kShapes[] = {
     {3, 8, 8, 1, 1, 256},   // 23: 8-direction sets, 8^3 bricks, 32 cells per pass
     {3, 4, 8, 1, 1, 128},   // 24: 8-direction sets, 4^3 bricks
     {3, 8, 4, 1, 1, 256},   // 25: 4-direction sets, 8^3 bricks, 64 cells per pass
+    {3, 8, 8, 1, 2, 512},   // 26: 8-direction sets, 2 octants share a pass (32 cells each)
+    {3, 8, 4, 1, 4, 512},   // 27: 4-direction sets, 4 octants share a pass (32 cells each)
+    {3, 4, 4, 1, 8, 256},   // 28: 4-direction sets, 4^3 bricks, 8 octants share a pass
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
@@ -637,7 +640,9 @@ constexpr size_t
 shmem_bytes()
 {
   constexpr Shape S = kShapes[SHAPE];
-  return Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>::BYTES;
+  constexpr size_t bytes = Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>::BYTES;
+  static_assert(bytes <= 160 * 1024, "a kernel shape must fit the 160 KiB LDS of a CDNA4 CU");
+  return bytes;
 }
 
 // Launch (op 0) or occupancy query (op 1) of one instantiated kernel.

@@ -92,6 +92,10 @@ SWEEP_SHAPES = [
     (24, 3, 10, 9, 7, 4, 4, capi.SCHEME_TW_DD),
     (25, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
     (25, 3, 9, 10, 11, 2, 3, capi.SCHEME_DD),
+    (26, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
+    (27, 3, 13, 9, 11, 4, 4, capi.SCHEME_TW_DD),
+    (28, 3, 17, 16, 19, 4, 4, capi.SCHEME_DD),
+    (28, 3, 10, 9, 7, 2, 2, capi.SCHEME_TW_DD),
     (16, 3, 12, 10, 9, 8, 4, capi.SCHEME_TW_DD),
 ]
 
