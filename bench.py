@@ -129,7 +129,7 @@ def batch_mode(args):
 
 
 def learned_compare(workload="high_scatter", n=64, groups=8, dd=True, every=5, reference=False,
-                    learned_only=False):
+                    learned_only=False, weights_path=None):
     """Learned-corrected FP32-sweep SI against plain FP64 GPU SI on the same
     high-scattering problem, both to a 1e-6 residual, optionally both against
     plain FP64 SI converged to 1e-9 (no corrections).  With dd the cells use
@@ -142,7 +142,7 @@ def learned_compare(workload="high_scatter", n=64, groups=8, dd=True, every=5, r
     from doma_amd import workloads
     from doma_amd.capi import FP32_SWEEP, FP64, SCHEME_DD, Context
 
-    weights = np.load(ROOT / "doma_amd" / "weights" / "si_correction_unet.npy")
+    weights = np.load(weights_path or ROOT / "doma_amd" / "weights" / "si_correction_unet.npy")
     p = workloads.diffusive(n) if workload == "diffusive" else workloads.high_scatter(n, groups)
     if dd:
         p.scheme = SCHEME_DD
@@ -258,7 +258,8 @@ def precision_mode(args):
 def learned_mode(args):
     print(json.dumps(learned_compare(args.workload if args.workload in ("diffusive", "high_scatter")
                                      else "high_scatter", args.n, args.groups, args.dd, args.every,
-                                     not args.no_reference, args.learned_only)), flush=True)
+                                     not args.no_reference, args.learned_only,
+                                     args.weights)), flush=True)
 
 
 def copy_calibration(nbytes=256 << 20, reps=3):
@@ -291,6 +292,7 @@ def main():
     ap.add_argument("--dd", action="store_true", help="diamond difference (learned mode)")
     ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
     ap.add_argument("--learned-only", action="store_true", help="learned mode: accelerated run only")
+    ap.add_argument("--weights", default=None, help="learned mode: correction weights (.npy)")
     ap.add_argument("--count", type=int, default=512, help="problems (batch mode)")
     ap.add_argument("--sample-cap", type=int, default=8, help="snapshots per problem and group")
     ap.add_argument("--workload", choices=["random_blocks", "multigroup_full", "high_scatter",

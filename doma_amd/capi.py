@@ -124,6 +124,15 @@ SIGNATURES = [
     ("doma_set_warm_start", C.c_int, [C.c_void_p, C.c_int32]),
     ("doma_net_forward", C.c_int, [C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32,
                                    C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("doma_net_trainer_create", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_float), C.c_int64,
+                                          C.POINTER(C.c_void_p)]),
+    ("doma_net_trainer_destroy", C.c_int, [C.c_void_p]),
+    ("doma_net_trainer_step", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.c_float, C.POINTER(C.c_float)]),
+    ("doma_net_trainer_gradient", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float)]),
+    ("doma_net_trainer_get_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("doma_batch_create", C.c_int, [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("doma_batch_destroy", C.c_int, [C.c_void_p]),
     ("doma_batch_set_materials", C.c_int, [C.c_void_p, C.c_int32, _ip, C.c_int32, _dp, _dp, _dp,
@@ -491,6 +500,68 @@ class Batch:
         _check(lib().doma_batch_get_sample(self.h, p, g, k, r.ctypes.data_as(fp),
                                            t.ctypes.data_as(fp)), "doma_batch_get_sample")
         return r, t
+
+
+class NetTrainer:
+    """Hand-written GPU training of the correction network (native kernels)."""
+
+    def __init__(self, params, crop: int = 32, batch: int = 8):
+        self.crop, self.batch = crop, batch
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        h = C.c_void_p()
+        fp = C.POINTER(C.c_float)
+        _check(lib().doma_net_trainer_create(crop, batch, params.ctypes.data_as(fp), params.size,
+                                             C.byref(h)), "doma_net_trainer_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().doma_net_trainer_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _io(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        n = self.batch * self.crop ** 3
+        if f.size != 3 * n or t.size != n:
+            raise ValueError("features must be [batch, crop, crop, crop, 3], targets [batch, crop^3]")
+        return f, t
+
+    def step(self, features, targets, lr: float) -> float:
+        f, t = self._io(features, targets)
+        fp = C.POINTER(C.c_float)
+        loss = C.c_float()
+        _check(lib().doma_net_trainer_step(self.h, f.ctypes.data_as(fp), t.ctypes.data_as(fp), lr,
+                                           C.byref(loss)), "doma_net_trainer_step")
+        return loss.value
+
+    def gradient(self, features, targets):
+        f, t = self._io(features, targets)
+        fp = C.POINTER(C.c_float)
+        loss = C.c_float()
+        g = np.zeros(net_param_count(), dtype=np.float32)
+        _check(lib().doma_net_trainer_gradient(self.h, f.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                               C.byref(loss), g.ctypes.data_as(fp)),
+               "doma_net_trainer_gradient")
+        return loss.value, g
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(net_param_count(), dtype=np.float32)
+        _check(lib().doma_net_trainer_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))),
+               "doma_net_trainer_get_params")
+        return p
 
 
 def media_generate(spec: MediaSpec):

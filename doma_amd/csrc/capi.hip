@@ -1277,3 +1277,67 @@ doma_set_warm_start(doma_ctx * c, int32_t enable)
   c->warm_start = enable ? 1 : 0;
   return DOMA_OK;
 }
+
+struct doma_net_trainer
+{
+  net::Trainer * t = nullptr;
+};
+
+extern "C" int
+doma_net_trainer_create(int32_t crop, int32_t batch, const float * params, int64_t count,
+                        doma_net_trainer ** out)
+{
+  if (!out || !params || count != net::total_params())
+    return fail(DOMA_ERR_INVALID, "doma_net_trainer_create: bad arguments");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(DOMA_ERR_DEVICE, "no HIP device available");
+  auto * h = new (std::nothrow) doma_net_trainer();
+  if (!h)
+    return fail(DOMA_ERR_INVALID, "out of host memory");
+  const int rc = net::trainer_create(crop, batch, params, &h->t);
+  if (rc)
+  {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_net_trainer_destroy(doma_net_trainer * h)
+{
+  if (!h)
+    return DOMA_OK;
+  net::trainer_free(h->t);
+  delete h;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_net_trainer_step(doma_net_trainer * h, const float * features, const float * targets,
+                      float lr, float * loss)
+{
+  if (!h || !features || !targets || !(lr >= 0.f))
+    return fail(DOMA_ERR_INVALID, "doma_net_trainer_step: bad arguments");
+  return net::trainer_run(h->t, features, targets, lr, true, loss, nullptr);
+}
+
+extern "C" int
+doma_net_trainer_gradient(doma_net_trainer * h, const float * features, const float * targets,
+                          float * loss, float * grad)
+{
+  if (!h || !features || !targets)
+    return fail(DOMA_ERR_INVALID, "doma_net_trainer_gradient: bad arguments");
+  return net::trainer_run(h->t, features, targets, 0.f, false, loss, grad);
+}
+
+extern "C" int
+doma_net_trainer_get_params(doma_net_trainer * h, float * params)
+{
+  if (!h || !params)
+    return fail(DOMA_ERR_INVALID, "doma_net_trainer_get_params: bad arguments");
+  return net::trainer_params(h->t, params);
+}

@@ -92,5 +92,18 @@ int net_apply(Net & n, const double * dlast, const int32_t * mat, const double *
 // Forward pass on given features (testing): feat FP32 [V][3] -> raw network
 // output FP32 [V] (no scale, no gain).
 int net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream_t s);
+
+// Hand-written training on batches of `batch` cubic crops of edge `crop`
+// (stacked along z): BF16 MFMA forward and data gradients, FP32 weight
+// gradients, relative-L2 loss sum (p - y)^2 / sum y^2, global-norm clipping at 1
+// and Adam (0.9, 0.999, 1e-8) on FP32 master weights.
+struct Trainer;
+int trainer_create(int crop, int batch, const float * params, Trainer ** out);
+void trainer_free(Trainer * t);
+// feats [batch][crop^3][3], targets [batch][crop^3] on the host.  update =
+// false computes the loss and the gradient only (grad_out may be NULL).
+int trainer_run(Trainer * t, const float * feats, const float * targets, float lr, bool update,
+                float * loss, float * grad_out);
+int trainer_params(Trainer * t, float * params);
 } // namespace net
 } // namespace doma

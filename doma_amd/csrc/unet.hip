@@ -13,7 +13,9 @@
 // tensor is ever stored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <new>
 #include <cstring>
 
 #include "internal.h"
@@ -62,7 +64,8 @@ struct ConvArgs
   int ca;              // its channel stride (= cin_a_p)
   const uint16_t * b;  // coarse input [V/8][cb], read at the parent voxel (or null)
   int cb;
-  int D, H, W, V;
+  int D, H, W, V;      // D counts every sample of a batch stacked along z
+  int Ds;              // depth of one sample (zero padding at its z faces)
   const uint16_t * w;  // [cout16][kpad]
   const float * bias;  // [cout]
   int cout, kpad, cin_p;
@@ -79,7 +82,7 @@ conv3_kernel(ConvArgs p)
   const int row = lane & 15, kg = lane >> 4;
   const int base = (blockIdx.x * (RT / 64) + wave) * 16 * MT;
   const int H2 = p.H >> 1, W2 = p.W >> 1;
-  int vz[MT], vy[MT], vx[MT];
+  int vz[MT], vzl[MT], vy[MT], vx[MT];
   bool vin[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
@@ -90,6 +93,7 @@ conv3_kernel(ConvArgs p)
     vx[mt] = vv % p.W;
     vy[mt] = (vv / p.W) % p.H;
     vz[mt] = vv / (p.W * p.H);
+    vzl[mt] = vz[mt] % p.Ds;
   }
   v4f acc[MT][NT];
 #pragma unroll
@@ -112,8 +116,8 @@ conv3_kernel(ConvArgs p)
     for (int mt = 0; mt < MT; ++mt)
     {
       v4u raw = {0u, 0u, 0u, 0u};
-      const int z = vz[mt] + dz, y = vy[mt] + dy, x = vx[mt] + dx;
-      if (tap < 27 && vin[mt] && z >= 0 && z < p.D && y >= 0 && y < p.H && x >= 0 && x < p.W)
+      const int z = vz[mt] + dz, zl = vzl[mt] + dz, y = vy[mt] + dy, x = vx[mt] + dx;
+      if (tap < 27 && vin[mt] && zl >= 0 && zl < p.Ds && y >= 0 && y < p.H && x >= 0 && x < p.W)
       {
         if (ci < p.ca)
           raw = *reinterpret_cast<const v4u *>(p.a + ((size_t)(z * p.H + y) * p.W + x) * p.ca + ci);
@@ -290,8 +294,24 @@ nblk(size_t n, int t)
 }
 
 int
+launch_conv(const ConvArgs & p, int cout16, hipStream_t s)
+{
+  constexpr int MT = 2;
+  const unsigned grid = nblk(p.V, (RT / 64) * 16 * MT);
+  if (cout16 == 16)
+    hipLaunchKernelGGL((conv3_kernel<1, MT>), dim3(grid), dim3(RT), 0, s, p);
+  else if (cout16 == 32)
+    hipLaunchKernelGGL((conv3_kernel<2, MT>), dim3(grid), dim3(RT), 0, s, p);
+  else if (cout16 == 64)
+    hipLaunchKernelGGL((conv3_kernel<4, MT>), dim3(grid), dim3(RT), 0, s, p);
+  else
+    return fail(DOMA_ERR_INVALID, "unsupported layer width");
+  return DOMA_OK;
+}
+
+int
 conv(const PackedLayer & L, const uint16_t * a, const uint16_t * b, int D, int H, int W,
-     uint16_t * out, int out_stride, hipStream_t s)
+     uint16_t * out, int out_stride, hipStream_t s, int Ds = 0)
 {
   ConvArgs p{};
   p.a = a;
@@ -302,6 +322,7 @@ conv(const PackedLayer & L, const uint16_t * a, const uint16_t * b, int D, int H
   p.H = H;
   p.W = W;
   p.V = D * H * W;
+  p.Ds = Ds > 0 ? Ds : D;
   p.w = L.w;
   p.bias = L.bias;
   p.cout = L.cout;
@@ -310,15 +331,7 @@ conv(const PackedLayer & L, const uint16_t * a, const uint16_t * b, int D, int H
   p.out = out;
   p.out_stride = out_stride;
   p.relu = L.relu;
-  constexpr int MT = 2;
-  const unsigned grid = nblk(p.V, (RT / 64) * 16 * MT);
-  if (L.cout16 == 16)
-    hipLaunchKernelGGL((conv3_kernel<1, MT>), dim3(grid), dim3(RT), 0, s, p);
-  else if (L.cout16 == 32)
-    hipLaunchKernelGGL((conv3_kernel<2, MT>), dim3(grid), dim3(RT), 0, s, p);
-  else
-    return fail(DOMA_ERR_INVALID, "unsupported layer width");
-  return DOMA_OK;
+  return launch_conv(p, L.cout16, s);
 }
 
 int
@@ -330,25 +343,27 @@ pool(const uint16_t * in, uint16_t * out, int D, int H, int W, int C, hipStream_
 }
 
 // Layers 0-9 on the features in n.feat; leaves the last hidden layer in n.d1b.
+// Ds: depth of one sample when a batch is stacked along z (0: one sample).
 int
-body(Net & n, hipStream_t s)
+body(Net & n, hipStream_t s, int Ds = 0)
 {
   const int D = n.D, H = n.H, W = n.W;
   const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D4 = D / 4, H4 = H / 4, W4 = W / 4;
   const int s1 = pad8(C1), s2 = pad8(C2), s3 = pad8(C3);
+  const int z1 = Ds > 0 ? Ds : D, z2 = z1 / 2, z4 = z1 / 4;
   int rc;
-  if ((rc = conv(n.layer[0], n.feat, nullptr, D, H, W, n.e1a, s1, s)) ||
-      (rc = conv(n.layer[1], n.e1a, nullptr, D, H, W, n.e1b, s1, s)) ||
+  if ((rc = conv(n.layer[0], n.feat, nullptr, D, H, W, n.e1a, s1, s, z1)) ||
+      (rc = conv(n.layer[1], n.e1a, nullptr, D, H, W, n.e1b, s1, s, z1)) ||
       (rc = pool(n.e1b, n.p1, D, H, W, s1, s)) ||
-      (rc = conv(n.layer[2], n.p1, nullptr, D2, H2, W2, n.e2a, s2, s)) ||
-      (rc = conv(n.layer[3], n.e2a, nullptr, D2, H2, W2, n.e2b, s2, s)) ||
+      (rc = conv(n.layer[2], n.p1, nullptr, D2, H2, W2, n.e2a, s2, s, z2)) ||
+      (rc = conv(n.layer[3], n.e2a, nullptr, D2, H2, W2, n.e2b, s2, s, z2)) ||
       (rc = pool(n.e2b, n.p2, D2, H2, W2, s2, s)) ||
-      (rc = conv(n.layer[4], n.p2, nullptr, D4, H4, W4, n.ba, s3, s)) ||
-      (rc = conv(n.layer[5], n.ba, nullptr, D4, H4, W4, n.bb, s3, s)) ||
-      (rc = conv(n.layer[6], n.e2b, n.bb, D2, H2, W2, n.d2a, s2, s)) ||
-      (rc = conv(n.layer[7], n.d2a, nullptr, D2, H2, W2, n.d2b, s2, s)) ||
-      (rc = conv(n.layer[8], n.e1b, n.d2b, D, H, W, n.d1a, s1, s)) ||
-      (rc = conv(n.layer[9], n.d1a, nullptr, D, H, W, n.d1b, s1, s)))
+      (rc = conv(n.layer[4], n.p2, nullptr, D4, H4, W4, n.ba, s3, s, z4)) ||
+      (rc = conv(n.layer[5], n.ba, nullptr, D4, H4, W4, n.bb, s3, s, z4)) ||
+      (rc = conv(n.layer[6], n.e2b, n.bb, D2, H2, W2, n.d2a, s2, s, z2)) ||
+      (rc = conv(n.layer[7], n.d2a, nullptr, D2, H2, W2, n.d2b, s2, s, z2)) ||
+      (rc = conv(n.layer[8], n.e1b, n.d2b, D, H, W, n.d1a, s1, s, z1)) ||
+      (rc = conv(n.layer[9], n.d1a, nullptr, D, H, W, n.d1b, s1, s, z1)))
     return rc;
   return DOMA_OK;
 }
@@ -504,5 +519,748 @@ net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream
                      n.layer[10].bias, N, out_dev);
   return DOMA_OK;
 }
+
+// ======================================================================== //
+// Training: forward on a batch of cubic crops stacked along z, hand-written //
+// backward (data gradients on the same MFMA convolution with flipped and    //
+// transposed weights, weight gradients by a deterministic chunked FP32      //
+// reduction), relative-L2 loss, global-norm clipping and Adam on FP32       //
+// master weights.                                                           //
+// ======================================================================== //
+namespace
+{
+constexpr int WG_CHUNK = 1024;  // voxels per weight-gradient partial
+
+// Slot of input channel ci of a layer in its padded concat layout.
+__device__ inline int
+in_slot(int ci, int cin_a, int ca_p)
+{
+  return ci < cin_a ? ci : ca_p + (ci - cin_a);
+}
+
+// FP32 layer weights W[cout][27][cin] -> forward BF16 [cout16][kpad], K = tap cin_p + slot.
+__global__ void
+pack_fwd_kernel(const float * W, int cout, int cin_a, int cin_b, int ca_p, int cb_p, int kpad,
+                int cout16, uint16_t * out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cout16 * kpad)
+    return;
+  const int co = i / kpad, k = i - co * kpad;
+  const int cin_p = ca_p + cb_p, cin = cin_a + cin_b;
+  const int tap = k / cin_p, slot = k - tap * cin_p;
+  int ci = -1;
+  if (slot < ca_p)
+    ci = slot < cin_a ? slot : -1;
+  else
+    ci = slot - ca_p < cin_b ? cin_a + (slot - ca_p) : -1;
+  float v = 0.f;
+  if (co < cout && tap < 27 && ci >= 0)
+    v = W[((size_t)co * 27 + tap) * cin + ci];
+  out[i] = f2bf(v);
+}
+
+// Data-gradient weights: rows = the forward layer's padded input slots, K =
+// tap' cout_p + co with tap' = 26 - tap (the stencil mirrored).
+__global__ void
+pack_bwd_kernel(const float * W, int cout, int cin_a, int cin_b, int ca_p, int cb_p, int cout_p,
+                int kpad, int rows16, uint16_t * out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows16 * kpad)
+    return;
+  const int r = i / kpad, k = i - r * kpad;
+  const int cin = cin_a + cin_b;
+  const int tapp = k / cout_p, co = k - tapp * cout_p;
+  int ci = -1;
+  if (r < ca_p)
+    ci = r < cin_a ? r : -1;
+  else if (r < ca_p + cb_p)
+    ci = r - ca_p < cin_b ? cin_a + (r - ca_p) : -1;
+  float v = 0.f;
+  if (tapp < 27 && co < cout && ci >= 0)
+    v = W[((size_t)co * 27 + (26 - tapp)) * cin + ci];
+  out[i] = f2bf(v);
+}
+
+__global__ void
+copy_f32_kernel(const float * in, int n, float * out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = in[i];
+}
+
+struct WgArgs
+{
+  const void * dy;     // BF16 [V][dy_stride] or FP32 [V] (head)
+  int dy_stride;
+  int dy_f32;
+  const uint16_t * a;  // forward input, full resolution [V][ca_p]
+  int ca_p;
+  const uint16_t * b;  // forward input, coarse [V/8][cb_p] (or null)
+  int cb_p;
+  int cin_a, cin_b, cout, taps;
+  int D, H, W, V, Ds;
+  int nchunks, nj;
+};
+
+// partial[chunk][j], j = flat index of (co, tap, ci) then the biases.
+__global__ __launch_bounds__(256) void
+wgrad_kernel(WgArgs p, float * partial)
+{
+  const int j = blockIdx.y * 256 + threadIdx.x;
+  const int chunk = blockIdx.x;
+  if (j >= p.nj)
+    return;
+  const int cin = p.cin_a + p.cin_b;
+  const int nw = p.cout * p.taps * cin;
+  int co, tap = -1, ci = 0;
+  if (j < nw)
+  {
+    co = j / (p.taps * cin);
+    const int r = j - co * p.taps * cin;
+    tap = r / cin;
+    ci = r - tap * cin;
+  }
+  else
+    co = j - nw;
+  const int slot = in_slot(ci, p.cin_a, p.ca_p);
+  int dz = 0, dy = 0, dx = 0;
+  if (p.taps == 27 && tap >= 0)
+  {
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+  }
+  const int H2 = p.H >> 1, W2 = p.W >> 1;
+  const int v0 = chunk * WG_CHUNK, v1 = min(p.V, v0 + WG_CHUNK);
+  float acc = 0.f;
+  for (int v = v0; v < v1; ++v)
+  {
+    const float g = p.dy_f32 ? reinterpret_cast<const float *>(p.dy)[v]
+                             : bf2f(reinterpret_cast<const uint16_t *>(p.dy)[(size_t)v * p.dy_stride + co]);
+    float x = 1.f;
+    if (tap >= 0)
+    {
+      const int vx = v % p.W, vy = (v / p.W) % p.H, vz = v / (p.W * p.H);
+      const int zl = vz % p.Ds + dz, z = vz + dz, y = vy + dy, xx = vx + dx;
+      x = 0.f;
+      if (zl >= 0 && zl < p.Ds && y >= 0 && y < p.H && xx >= 0 && xx < p.W)
+      {
+        if (slot < p.ca_p)
+          x = bf2f(p.a[((size_t)(z * p.H + y) * p.W + xx) * p.ca_p + slot]);
+        else
+          x = bf2f(p.b[((size_t)((z >> 1) * H2 + (y >> 1)) * W2 + (xx >> 1)) * p.cb_p +
+                       (slot - p.ca_p)]);
+      }
+    }
+    acc = fmaf(g, x, acc);
+  }
+  partial[(size_t)chunk * p.nj + j] = acc;
+}
+
+__global__ void
+wgrad_final_kernel(const float * partial, int nchunks, int nj, float * grad)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj)
+    return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; ++c)
+    s += partial[(size_t)c * nj + j];
+  grad[j] = s;
+}
+
+// Loss terms: block partials of sum (p - y)^2 and sum y^2.
+__global__ __launch_bounds__(256) void
+loss_partial_kernel(const float * pred, const float * y, int V, float * red)
+{
+  __shared__ float s1[256], s2[256];
+  float a = 0.f, b = 0.f;
+  for (int v = blockIdx.x * 256 + threadIdx.x; v < V; v += gridDim.x * 256)
+  {
+    const float d = pred[v] - y[v];
+    a += d * d;
+    b += y[v] * y[v];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1)
+  {
+    if ((int)threadIdx.x < k)
+    {
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+  {
+    red[2 * blockIdx.x] = s1[0];
+    red[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+// out[0] = loss = sum (p-y)^2 / (sum y^2 + eps), out[1] = the denominator.
+__global__ __launch_bounds__(256) void
+loss_final_kernel(const float * red, int nb, float * out)
+{
+  __shared__ float s1[256], s2[256];
+  float a = 0.f, b = 0.f;
+  for (int k = threadIdx.x; k < nb; k += 256)
+  {
+    a += red[2 * k];
+    b += red[2 * k + 1];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1)
+  {
+    if ((int)threadIdx.x < k)
+    {
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+  {
+    const float den = s2[0] + 1e-12f;
+    out[0] = s1[0] / den;
+    out[1] = den;
+  }
+}
+
+__global__ void
+dpred_kernel(const float * pred, const float * y, const float * lossv, int V, float * dpred)
+{
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < V)
+    dpred[v] = 2.f * (pred[v] - y[v]) / lossv[1];
+}
+
+// Gradient at the head's input, masked by the ReLU of the last hidden layer.
+__global__ void
+head_back_kernel(const float * dpred, const float * wh, const uint16_t * h, int C, int stride,
+                 int V, uint16_t * dy)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)V * stride)
+    return;
+  const int c = (int)(i % stride);
+  const size_t v = i / stride;
+  float g = 0.f;
+  if (c < C && bf2f(h[i]) > 0.f)
+    g = dpred[v] * wh[c];
+  dy[i] = f2bf(g);
+}
+
+// dy[v][c] = src[v][c] if act[v][c] > 0 else 0 (c < C; padded channels 0).
+template <typename T>
+__global__ void
+mask_kernel(const T * src, int src_stride, const uint16_t * act, int act_stride, int C, int V,
+            uint16_t * dy)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)V * act_stride)
+    return;
+  const int c = (int)(i % act_stride);
+  const size_t v = i / act_stride;
+  float g = 0.f;
+  if (c < C && bf2f(act[i]) > 0.f)
+  {
+    if constexpr (sizeof(T) == 2)
+      g = bf2f(src[v * src_stride + c]);
+    else
+      g = src[v * src_stride + c];
+  }
+  dy[i] = f2bf(g);
+}
+
+// Skip part of a concat data gradient: acc[v][c] = g[v][c] (c < C).
+__global__ void
+split_skip_kernel(const uint16_t * g, int g_stride, int C, int acc_stride, int V, float * acc)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)V * acc_stride)
+    return;
+  const int c = (int)(i % acc_stride);
+  const size_t v = i / acc_stride;
+  acc[i] = c < C ? bf2f(g[v * g_stride + c]) : 0.f;
+}
+
+// Upsampling backward: low[u][c] = sum over the 8 children of g[child][off + c].
+__global__ void
+up_back_kernel(const uint16_t * g, int g_stride, int off, int C, int low_stride, int D, int H,
+               int W, float * low)
+{
+  const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)D2 * H2 * W2 * low_stride)
+    return;
+  const int c = (int)(i % low_stride);
+  const size_t u = i / low_stride;
+  float s = 0.f;
+  if (c < C)
+  {
+    const int x = (int)(u % W2), y = (int)((u / W2) % H2), z = (int)(u / ((size_t)W2 * H2));
+    for (int d = 0; d < 8; ++d)
+    {
+      const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
+      s += bf2f(g[((size_t)(zz * H + yy) * W + xx) * g_stride + off + c]);
+    }
+  }
+  low[i] = s;
+}
+
+// Average-pooling backward, accumulated: acc[child][c] += g[parent][c] / 8.
+__global__ void
+pool_back_add_kernel(const uint16_t * g, int g_stride, int C, int acc_stride, int D, int H, int W,
+                     float * acc)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)D * H * W * acc_stride)
+    return;
+  const int c = (int)(i % acc_stride);
+  if (c >= C)
+    return;
+  const size_t v = i / acc_stride;
+  const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((size_t)W * H));
+  const size_t u = ((size_t)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
+  acc[i] += 0.125f * bf2f(g[u * g_stride + c]);
+}
+
+__global__ __launch_bounds__(256) void
+sumsq_f32_kernel(const float * g, int n, float * red)
+{
+  __shared__ float sh[256];
+  float a = 0.f;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    a += g[i] * g[i];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1)
+  {
+    if ((int)threadIdx.x < k)
+      sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    red[blockIdx.x] = sh[0];
+}
+
+// Adam with global-norm clipping (max norm 1): red holds nb partial squared norms.
+__global__ void
+adam_kernel(float * w, const float * g, float * m, float * v, int n, const float * red, int nb,
+            float lr, float b1, float b2, float eps, int step)
+{
+  __shared__ float scale;
+  if (threadIdx.x == 0)
+  {
+    float s = 0.f;
+    for (int k = 0; k < nb; ++k)
+      s += red[k];
+    const float norm = sqrtf(s);
+    scale = norm > 1.f ? 1.f / (norm + 1e-6f) : 1.f;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const float gi = g[i] * scale;
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - powf(b1, (float)step));
+  const float vh = vi / (1.f - powf(b2, (float)step));
+  w[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+} // namespace
+
+struct Trainer
+{
+  int C = 0, nb = 0;
+  Net net;
+  float * params = nullptr;  // FP32 master weights [P]
+  float * grads = nullptr;
+  float * m = nullptr;
+  float * v = nullptr;
+  PackedLayer bwd[kLayers];  // data-gradient weights (layers 1-9)
+  float * zero = nullptr;     // [64]
+  float * fin = nullptr;      // [V][3]
+  float * y = nullptr;        // [V]
+  float * pred = nullptr;     // [V]
+  float * dpred = nullptr;    // [V]
+  uint16_t * dyA = nullptr;   // [V][64]
+  uint16_t * dyB = nullptr;   // [V][64]
+  uint16_t * gtmp = nullptr;  // [V][64]
+  float * acc1 = nullptr;     // [V][16]
+  float * acc2 = nullptr;     // [V/8][24]
+  float * low = nullptr;      // [V/8][32]
+  float * partial = nullptr;
+  size_t partial_cap = 0;
+  float * red = nullptr;      // [1024]
+  float * lossv = nullptr;    // [2]
+  int64_t off[kLayers]{};     // flat offset of each layer
+  int step = 0;
+};
+
+namespace
+{
+template <typename T>
+int
+alloc0(T ** p, size_t n)
+{
+  *p = nullptr;
+  DOMA_HIP(hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)));
+  DOMA_HIP(hipMemset(*p, 0, n * sizeof(T)));
+  return DOMA_OK;
+}
+
+int
+repack(Trainer & t, hipStream_t s)
+{
+  for (int l = 0; l < kLayers; ++l)
+  {
+    const LayerSpec & sp = kSpec[l];
+    PackedLayer & L = t.net.layer[l];
+    const float * W = t.params + t.off[l];
+    const int cin = sp.cin_a + sp.cin_b;
+    if (sp.taps == 1)
+    {
+      hipLaunchKernelGGL(copy_f32_kernel, dim3(1), dim3(64), 0, s, W, cin + 1, L.bias);
+      continue;
+    }
+    hipLaunchKernelGGL(pack_fwd_kernel, dim3(nblk((size_t)L.cout16 * L.kpad, 256)), dim3(256), 0,
+                       s, W, sp.cout, sp.cin_a, sp.cin_b, L.cin_a_p, L.cin_b_p, L.kpad, L.cout16,
+                       L.w);
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(1), dim3(64), 0, s,
+                       W + (int64_t)sp.cout * 27 * cin, sp.cout, L.bias);
+    if (l == 0)
+      continue;
+    PackedLayer & B = t.bwd[l];
+    hipLaunchKernelGGL(pack_bwd_kernel, dim3(nblk((size_t)B.cout16 * B.kpad, 256)), dim3(256), 0,
+                       s, W, sp.cout, sp.cin_a, sp.cin_b, L.cin_a_p, L.cin_b_p, B.cin_a_p, B.kpad,
+                       B.cout16, B.w);
+  }
+  return DOMA_OK;
+}
+
+// Weight and bias gradients of layer l into t.grads.
+int
+wgrad(Trainer & t, int l, const void * dy, int dy_stride, bool dy_f32, const uint16_t * a,
+      const uint16_t * b, int D, int H, int W, int Ds, hipStream_t s)
+{
+  const LayerSpec & sp = kSpec[l];
+  const PackedLayer & L = t.net.layer[l];
+  WgArgs p{};
+  p.dy = dy;
+  p.dy_stride = dy_stride;
+  p.dy_f32 = dy_f32 ? 1 : 0;
+  p.a = a;
+  p.ca_p = sp.taps == 1 ? pad8(sp.cin_a) : L.cin_a_p;
+  p.b = b;
+  p.cb_p = L.cin_b_p;
+  p.cin_a = sp.cin_a;
+  p.cin_b = sp.cin_b;
+  p.cout = sp.cout;
+  p.taps = sp.taps;
+  p.D = D;
+  p.H = H;
+  p.W = W;
+  p.V = D * H * W;
+  p.Ds = Ds;
+  p.nchunks = (p.V + WG_CHUNK - 1) / WG_CHUNK;
+  p.nj = (int)layer_params(sp);
+  const size_t need = (size_t)p.nchunks * p.nj;
+  if (need > t.partial_cap)
+    return fail(DOMA_ERR_INVALID, "weight-gradient workspace too small");
+  hipLaunchKernelGGL(wgrad_kernel, dim3(p.nchunks, nblk(p.nj, 256)), dim3(256), 0, s, p,
+                     t.partial);
+  hipLaunchKernelGGL(wgrad_final_kernel, dim3(nblk(p.nj, 256)), dim3(256), 0, s, t.partial,
+                     p.nchunks, p.nj, t.grads + t.off[l]);
+  return DOMA_OK;
+}
+
+// Data gradient of layer l: out[v][slot] over the forward input's padded slots.
+int
+dgrad(Trainer & t, int l, const uint16_t * dy, int D, int H, int W, int Ds, uint16_t * out,
+      int out_stride, hipStream_t s)
+{
+  const PackedLayer & B = t.bwd[l];
+  const PackedLayer & L = t.net.layer[l];
+  ConvArgs p{};
+  p.a = dy;
+  p.ca = B.cin_a_p;
+  p.b = nullptr;
+  p.cb = 0;
+  p.D = D;
+  p.H = H;
+  p.W = W;
+  p.V = D * H * W;
+  p.Ds = Ds;
+  p.w = B.w;
+  p.bias = t.zero;
+  p.cout = L.cin_a_p + L.cin_b_p;
+  p.kpad = B.kpad;
+  p.cin_p = B.cin_a_p;
+  p.out = out;
+  p.out_stride = out_stride;
+  p.relu = 0;
+  return launch_conv(p, B.cout16, s);
+}
+
+template <typename T>
+int
+mask(const T * src, int src_stride, const uint16_t * act, int act_stride, int C, int V,
+     uint16_t * dy, hipStream_t s)
+{
+  hipLaunchKernelGGL((mask_kernel<T>), dim3(nblk((size_t)V * act_stride, 256)), dim3(256), 0, s,
+                     src, src_stride, act, act_stride, C, V, dy);
+  return DOMA_OK;
+}
+
+int
+forward(Trainer & t, hipStream_t s)
+{
+  Net & n = t.net;
+  const int V = n.D * n.H * n.W;
+  hipLaunchKernelGGL(features_f32_kernel, dim3(nblk(V, 256)), dim3(256), 0, s, t.fin, V, n.feat);
+  int rc = body(n, s, t.C);
+  if (rc)
+    return rc;
+  hipLaunchKernelGGL(head_raw_kernel, dim3(nblk(V, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
+                     n.layer[10].bias, V, t.pred);
+  return DOMA_OK;
+}
+
+int
+backward(Trainer & t, hipStream_t s)
+{
+  Net & n = t.net;
+  const int D = n.D, H = n.H, W = n.W, V = D * H * W;
+  const int D2 = D / 2, H2 = H / 2, W2 = W / 2, V2 = V / 8, D4 = D / 4, H4 = H / 4, W4 = W / 4;
+  const int V4 = V / 64;
+  const int z1 = t.C, z2 = t.C / 2, z4 = t.C / 4;
+  const int s1 = pad8(C1), s2 = pad8(C2), s3 = pad8(C3);
+  int rc;
+  hipLaunchKernelGGL(loss_partial_kernel, dim3(256), dim3(256), 0, s, t.pred, t.y, V, t.red);
+  hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(256), 0, s, t.red, 256, t.lossv);
+  hipLaunchKernelGGL(dpred_kernel, dim3(nblk(V, 256)), dim3(256), 0, s, t.pred, t.y, t.lossv, V,
+                     t.dpred);
+  // head (1x1x1)
+  if ((rc = wgrad(t, 10, t.dpred, 1, true, n.d1b, nullptr, D, H, W, z1, s)))
+    return rc;
+  hipLaunchKernelGGL(head_back_kernel, dim3(nblk((size_t)V * s1, 256)), dim3(256), 0, s, t.dpred,
+                     t.params + t.off[10], n.d1b, C1, s1, V, t.dyA);
+  // dec1b
+  if ((rc = wgrad(t, 9, t.dyA, s1, false, n.d1a, nullptr, D, H, W, z1, s)) ||
+      (rc = dgrad(t, 9, t.dyA, D, H, W, z1, t.gtmp, s1, s)) ||
+      (rc = mask(t.gtmp, s1, n.d1a, s1, C1, V, t.dyB, s)))
+    return rc;
+  // dec1a: concat(e1b, up(d2b))
+  const int c8 = s1 + s2;  // 16 + 24 padded input slots
+  if ((rc = wgrad(t, 8, t.dyB, s1, false, n.e1b, n.d2b, D, H, W, z1, s)) ||
+      (rc = dgrad(t, 8, t.dyB, D, H, W, z1, t.gtmp, c8, s)))
+    return rc;
+  hipLaunchKernelGGL(split_skip_kernel, dim3(nblk((size_t)V * s1, 256)), dim3(256), 0, s, t.gtmp,
+                     c8, C1, s1, V, t.acc1);
+  hipLaunchKernelGGL(up_back_kernel, dim3(nblk((size_t)V2 * s2, 256)), dim3(256), 0, s, t.gtmp, c8,
+                     s1, C2, s2, D, H, W, t.low);
+  if ((rc = mask(t.low, s2, n.d2b, s2, C2, V2, t.dyA, s)))
+    return rc;
+  // dec2b
+  if ((rc = wgrad(t, 7, t.dyA, s2, false, n.d2a, nullptr, D2, H2, W2, z2, s)) ||
+      (rc = dgrad(t, 7, t.dyA, D2, H2, W2, z2, t.gtmp, s2, s)) ||
+      (rc = mask(t.gtmp, s2, n.d2a, s2, C2, V2, t.dyB, s)))
+    return rc;
+  // dec2a: concat(e2b, up(bb))
+  const int c6 = s2 + s3;  // 24 + 32
+  if ((rc = wgrad(t, 6, t.dyB, s2, false, n.e2b, n.bb, D2, H2, W2, z2, s)) ||
+      (rc = dgrad(t, 6, t.dyB, D2, H2, W2, z2, t.gtmp, c6, s)))
+    return rc;
+  hipLaunchKernelGGL(split_skip_kernel, dim3(nblk((size_t)V2 * s2, 256)), dim3(256), 0, s, t.gtmp,
+                     c6, C2, s2, V2, t.acc2);
+  hipLaunchKernelGGL(up_back_kernel, dim3(nblk((size_t)V4 * s3, 256)), dim3(256), 0, s, t.gtmp, c6,
+                     s2, C3, s3, D2, H2, W2, t.low);
+  if ((rc = mask(t.low, s3, n.bb, s3, C3, V4, t.dyA, s)))
+    return rc;
+  // bottleneck
+  if ((rc = wgrad(t, 5, t.dyA, s3, false, n.ba, nullptr, D4, H4, W4, z4, s)) ||
+      (rc = dgrad(t, 5, t.dyA, D4, H4, W4, z4, t.gtmp, s3, s)) ||
+      (rc = mask(t.gtmp, s3, n.ba, s3, C3, V4, t.dyB, s)) ||
+      (rc = wgrad(t, 4, t.dyB, s3, false, n.p2, nullptr, D4, H4, W4, z4, s)) ||
+      (rc = dgrad(t, 4, t.dyB, D4, H4, W4, z4, t.gtmp, s2, s)))
+    return rc;
+  hipLaunchKernelGGL(pool_back_add_kernel, dim3(nblk((size_t)V2 * s2, 256)), dim3(256), 0, s,
+                     t.gtmp, s2, C2, s2, D2, H2, W2, t.acc2);
+  if ((rc = mask(t.acc2, s2, n.e2b, s2, C2, V2, t.dyA, s)))
+    return rc;
+  // encoder level 2
+  if ((rc = wgrad(t, 3, t.dyA, s2, false, n.e2a, nullptr, D2, H2, W2, z2, s)) ||
+      (rc = dgrad(t, 3, t.dyA, D2, H2, W2, z2, t.gtmp, s2, s)) ||
+      (rc = mask(t.gtmp, s2, n.e2a, s2, C2, V2, t.dyB, s)) ||
+      (rc = wgrad(t, 2, t.dyB, s2, false, n.p1, nullptr, D2, H2, W2, z2, s)) ||
+      (rc = dgrad(t, 2, t.dyB, D2, H2, W2, z2, t.gtmp, s1, s)))
+    return rc;
+  hipLaunchKernelGGL(pool_back_add_kernel, dim3(nblk((size_t)V * s1, 256)), dim3(256), 0, s,
+                     t.gtmp, s1, C1, s1, D, H, W, t.acc1);
+  if ((rc = mask(t.acc1, s1, n.e1b, s1, C1, V, t.dyA, s)))
+    return rc;
+  // encoder level 1
+  if ((rc = wgrad(t, 1, t.dyA, s1, false, n.e1a, nullptr, D, H, W, z1, s)) ||
+      (rc = dgrad(t, 1, t.dyA, D, H, W, z1, t.gtmp, s1, s)) ||
+      (rc = mask(t.gtmp, s1, n.e1a, s1, C1, V, t.dyB, s)) ||
+      (rc = wgrad(t, 0, t.dyB, s1, false, n.feat, nullptr, D, H, W, z1, s)))
+    return rc;
+  return DOMA_OK;
+}
+} // namespace
+
+void
+trainer_free(Trainer * t)
+{
+  if (!t)
+    return;
+  net_free(t->net);
+  for (auto & B : t->bwd)
+  {
+    dfree(B.w);
+    dfree(B.bias);
+  }
+  dfree(t->params);
+  dfree(t->grads);
+  dfree(t->m);
+  dfree(t->v);
+  dfree(t->zero);
+  dfree(t->fin);
+  dfree(t->y);
+  dfree(t->pred);
+  dfree(t->dpred);
+  dfree(t->dyA);
+  dfree(t->dyB);
+  dfree(t->gtmp);
+  dfree(t->acc1);
+  dfree(t->acc2);
+  dfree(t->low);
+  dfree(t->partial);
+  dfree(t->red);
+  dfree(t->lossv);
+  delete t;
+}
+
+int
+trainer_create(int crop, int batch, const float * params, Trainer ** out)
+{
+  *out = nullptr;
+  if (crop < 4 || crop % 4 || batch < 1)
+    return fail(DOMA_ERR_INVALID, "trainer: crop must be a multiple of 4 and batch >= 1");
+  Trainer * t = new (std::nothrow) Trainer();
+  if (!t)
+    return fail(DOMA_ERR_INVALID, "out of host memory");
+  t->C = crop;
+  t->nb = batch;
+  int rc = net_setup(t->net, params, batch * crop, crop, crop);
+  const int64_t P = total_params();
+  const size_t V = (size_t)batch * crop * crop * crop;
+  int64_t o = 0;
+  for (int l = 0; l < kLayers; ++l)
+  {
+    t->off[l] = o;
+    o += layer_params(kSpec[l]);
+  }
+  size_t cap = 0;
+  for (int l = 0; l < kLayers; ++l)
+  {
+    const size_t vl = l <= 1 || l >= 8 ? V : (l == 4 || l == 5 ? V / 64 : V / 8);
+    cap = std::max(cap, ((vl + WG_CHUNK - 1) / WG_CHUNK) * (size_t)layer_params(kSpec[l]));
+  }
+  if (!rc)
+  {
+    if ((rc = alloc0(&t->params, P)) || (rc = alloc0(&t->grads, P)) || (rc = alloc0(&t->m, P)) ||
+        (rc = alloc0(&t->v, P)) || (rc = alloc0(&t->zero, 64)) || (rc = alloc0(&t->fin, V * 3)) ||
+        (rc = alloc0(&t->y, V)) || (rc = alloc0(&t->pred, V)) || (rc = alloc0(&t->dpred, V)) ||
+        (rc = alloc0(&t->dyA, V * 64)) || (rc = alloc0(&t->dyB, V * 64)) ||
+        (rc = alloc0(&t->gtmp, V * 64)) || (rc = alloc0(&t->acc1, V * 16)) ||
+        (rc = alloc0(&t->acc2, V / 8 * 24)) || (rc = alloc0(&t->low, V / 8 * 32)) ||
+        (rc = alloc0(&t->partial, cap)) || (rc = alloc0(&t->red, 1024)) ||
+        (rc = alloc0(&t->lossv, 2)))
+      ;
+    t->partial_cap = cap;
+  }
+  for (int l = 1; !rc && l < kLayers - 1; ++l)
+  {
+    const LayerSpec & sp = kSpec[l];
+    const PackedLayer & L = t->net.layer[l];
+    PackedLayer & B = t->bwd[l];
+    B.cin_a_p = pad8(sp.cout);
+    B.cin_b_p = 0;
+    const int rows = L.cin_a_p + L.cin_b_p;
+    B.cout = rows;
+    B.cout16 = rows <= 16 ? 16 : (rows <= 32 ? 32 : 64);
+    B.kpad = (27 * B.cin_a_p + 31) / 32 * 32;
+    rc = alloc0(&B.w, (size_t)B.cout16 * B.kpad);
+  }
+  if (!rc)
+  {
+    if (hipMemcpy(t->params, params, P * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(DOMA_ERR_DEVICE, "trainer: upload failed");
+  }
+  if (!rc)
+    rc = repack(*t, nullptr);
+  if (!rc && hipDeviceSynchronize() != hipSuccess)
+    rc = fail(DOMA_ERR_DEVICE, "trainer: set-up failed");
+  if (rc)
+  {
+    trainer_free(t);
+    return rc;
+  }
+  *out = t;
+  return DOMA_OK;
+}
+
+int
+trainer_run(Trainer * t, const float * feats, const float * targets, float lr, bool update,
+            float * loss, float * grad_out)
+{
+  const size_t V = (size_t)t->nb * t->C * t->C * t->C;
+  hipStream_t s = nullptr;
+  DOMA_HIP(hipMemcpy(t->fin, feats, V * 3 * sizeof(float), hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(t->y, targets, V * sizeof(float), hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = forward(*t, s)) || (rc = backward(*t, s)))
+    return rc;
+  const int P = (int)total_params();
+  if (update)
+  {
+    ++t->step;
+    hipLaunchKernelGGL(sumsq_f32_kernel, dim3(256), dim3(256), 0, s, t->grads, P, t->red);
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(P, 256)), dim3(256), 0, s, t->params, t->grads, t->m,
+                       t->v, P, t->red, 256, lr, 0.9f, 0.999f, 1e-8f, t->step);
+    if ((rc = repack(*t, s)))
+      return rc;
+  }
+  float lv[2];
+  DOMA_HIP(hipMemcpy(lv, t->lossv, 2 * sizeof(float), hipMemcpyDeviceToHost));
+  if (loss)
+    *loss = lv[0];
+  if (grad_out)
+    DOMA_HIP(hipMemcpy(grad_out, t->grads, (size_t)P * sizeof(float), hipMemcpyDeviceToHost));
+  DOMA_HIP(hipDeviceSynchronize());
+  DOMA_HIP(hipGetLastError());
+  return DOMA_OK;
+}
+
+int
+trainer_params(Trainer * t, float * params)
+{
+  DOMA_HIP(hipMemcpy(params, t->params, total_params() * sizeof(float), hipMemcpyDeviceToHost));
+  return DOMA_OK;
+}
+
 } // namespace net
 } // namespace doma

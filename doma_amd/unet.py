@@ -182,6 +182,60 @@ def train(samples, steps=2000, crops=(16, 32), batch_size=8, lr=1e-3, seed=0, de
     return flatten(model.cpu()), float(np.mean(errs)) if errs else float("nan")
 
 
+def init_params(seed=0):
+    """Fresh parameters without PyTorch: per layer W, b ~ U(-1/sqrt(fan_in),
+    1/sqrt(fan_in)) (the default initialisation of a PyTorch Conv3d)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for a, b, co, t in SPEC:
+        fan_in = (a + b) * t
+        bound = 1.0 / math.sqrt(fan_in)
+        out.append(rng.uniform(-bound, bound, co * t * (a + b)))
+        out.append(rng.uniform(-bound, bound, co))
+    return np.concatenate(out).astype(np.float32)
+
+
+def train_native(samples, steps=3000, crop=32, batch_size=8, lr=1e-3, seed=0, log=None,
+                 holdout=0.05, params=None):
+    """Training on the native hand-written kernels (BF16 MFMA forward and data
+    gradients, FP32 weight gradients, Adam): random crops of edge `crop` from
+    samples at least that large, cosine learning-rate decay.  Returns the flat
+    parameters and the held-out relative L2 error (native forward pass)."""
+    from . import capi
+
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(samples))
+    nh = max(1, int(holdout * len(samples)))
+    held = [samples[i] for i in order[:nh]]
+    fit = [samples[i] for i in order[nh:] if samples[i][0].shape[0] >= crop]
+    if not fit:
+        raise ValueError(f"no training sample is at least {crop} cells wide")
+    flat = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
+    c = crop
+    with capi.NetTrainer(flat, crop=c, batch=batch_size) as tr:
+        for step in range(steps):
+            xb = np.empty((batch_size, c, c, c, 3), dtype=np.float32)
+            yb = np.empty((batch_size, c, c, c), dtype=np.float32)
+            for k in range(batch_size):
+                f, t = fit[int(rng.integers(len(fit)))]
+                n = f.shape[0]
+                o = [int(rng.integers(0, n - c + 1)) for _ in range(3)]
+                xb[k] = f[o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c]
+                yb[k] = t[o[0]:o[0] + c, o[1]:o[1] + c, o[2]:o[2] + c]
+            cur = 0.5 * lr * (1.0 + math.cos(math.pi * step / steps))
+            loss = tr.step(xb, yb.reshape(batch_size, -1), cur)
+            if log and (step % 200 == 0 or step == steps - 1):
+                log(f"native train step {step} loss {loss:.4f}")
+        flat = tr.params()
+    errs = []
+    for f, t in held[:64]:
+        if min(f.shape[:3]) % 4:
+            continue
+        pr = capi.net_forward(flat, f)
+        errs.append(float(np.linalg.norm(pr - t) / (np.linalg.norm(t) + 1e-30)))
+    return flat, float(np.mean(errs)) if errs else float("nan")
+
+
 def training_problems(seed=20261210, small=False):
     """The batched data-generation set: uneven cubes with 1/2/4 groups and
     c ~ U[0.9, 0.99] (down-scatter), 8-group full-matrix high-scattering media
@@ -228,11 +282,13 @@ def load_samples(path):
 
 
 def main(argv=None):
-    """Training data and weights for the correction, in two processes (the
-    native library and PyTorch each load their own HIP runtime, and only one of
-    them can own the GPU in a process):
+    """Training data and weights for the correction:
       python -m doma_amd.unet data SAMPLES.npz [--small] [--sample-cap K]
-      python -m doma_amd.unet train SAMPLES.npz OUT.npy [--steps N]"""
+      python -m doma_amd.unet train SAMPLES.npz OUT.npy [--steps N] [--native]
+    --native trains with the library's own kernels.  Without it PyTorch trains
+    the twin network; then data and training must run in separate processes
+    (the library and PyTorch each load their own HIP runtime, and only one of
+    them can own the GPU in a process)."""
     import argparse
     import sys
     import time
@@ -244,6 +300,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--sample-cap", type=int, default=8)
+    ap.add_argument("--native", action="store_true",
+                    help="train with the library's hand-written kernels instead of PyTorch")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -272,8 +330,11 @@ def main(argv=None):
         log(f"data: {len(samples)} pairs written to {args.samples} in {time.time() - t0:.0f} s")
         return
     samples = load_samples(args.samples)
-    log(f"train: {len(samples)} pairs")
-    flat, held_err = train(samples, steps=args.steps, log=log)
+    log(f"train: {len(samples)} pairs" + (" (native kernels)" if args.native else " (PyTorch)"))
+    if args.native:
+        flat, held_err = train_native(samples, steps=args.steps, log=log)
+    else:
+        flat, held_err = train(samples, steps=args.steps, log=log)
     np.save(args.out, flat)
     log(f"held-out relative L2 error of the predicted correction: {held_err:.3f}")
     log(f"saved {args.out} ({flat.size} parameters) after {time.time() - t0:.0f} s")

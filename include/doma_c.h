@@ -180,6 +180,22 @@ int doma_set_warm_start(doma_ctx *ctx, int32_t enable);
 int doma_net_forward(const float *params, int64_t count, int32_t D, int32_t H, int32_t W,
                      const float *features, float *out);
 
+/* Hand-written training of the correction network on batches of `batch` cubic
+ * crops of edge `crop` (multiple of 4): BF16 MFMA forward and data gradients,
+ * FP32 weight gradients, loss sum (p - y)^2 / sum y^2, global-norm clipping at
+ * 1, Adam (0.9, 0.999, 1e-8) on FP32 master weights.  features are
+ * [batch][crop^3][3] and targets [batch][crop^3] (x fastest), on the host. */
+typedef struct doma_net_trainer doma_net_trainer;
+int doma_net_trainer_create(int32_t crop, int32_t batch, const float *params, int64_t count,
+                            doma_net_trainer **out);
+int doma_net_trainer_destroy(doma_net_trainer *t);
+int doma_net_trainer_step(doma_net_trainer *t, const float *features, const float *targets,
+                          float lr, float *loss);
+/* Loss and gradient [count] without an update (tests). */
+int doma_net_trainer_gradient(doma_net_trainer *t, const float *features, const float *targets,
+                              float *loss, float *grad);
+int doma_net_trainer_get_params(doma_net_trainer *t, float *params);
+
 /* ------------------------------------------------------------------------- */
 /* Batches: many independent 3D problems, one persistent sweep launch per      */
 /* source-iteration round (no reference counterpart).                          */

@@ -142,3 +142,72 @@ def test_warm_start_same_fixed_point():
     assert st0["success"] and st1["success"]
     assert st1["sweeps"] < st0["sweeps"]
     assert rel_err(phi1, phi0).max() < 1e-8
+
+
+def _torch_grad(flat, feats, targets):
+    """Loss sum (p - y)^2 / sum y^2 and its gradient by PyTorch autograd (FP32)."""
+    import torch
+    m = unet.unflatten(unet.make_model(), flat)
+    x = torch.from_numpy(feats).permute(0, 4, 1, 2, 3).contiguous()
+    y = torch.from_numpy(targets)[:, None]
+    pred = m(x)
+    loss = ((pred - y) ** 2).sum() / ((y ** 2).sum() + 1e-12)
+    loss.backward()
+    loss = loss.detach()
+    g = []
+    for conv in m.convs:
+        g.append(conv.weight.grad.permute(0, 2, 3, 4, 1).reshape(-1).numpy())
+        g.append(conv.bias.grad.numpy())
+    return float(loss), np.concatenate(g)
+
+
+@pytest.mark.gpu
+def test_native_gradient_matches_autograd():
+    """The hand-written backward pass (BF16 MFMA data gradients, FP32 weight
+    gradients) against PyTorch FP32 autograd with the same BF16-rounded weights:
+    per layer, relative L2 error of the weight gradient < 5e-2."""
+    torch = pytest.importorskip("torch")
+    torch.manual_seed(7)
+    flat = unet.bf16_round(unet.flatten(unet.make_model()))
+    rng = np.random.default_rng(8)
+    B, Cr = 2, 16
+    # inputs exactly representable in BF16, as the kernels store them
+    f = unet.bf16_round(rng.standard_normal((B, Cr, Cr, Cr, 3))).astype(np.float32)
+    y = rng.standard_normal((B, Cr, Cr, Cr)).astype(np.float32)
+    print("native gradient", flush=True)
+    with capi.NetTrainer(flat, crop=Cr, batch=B) as tr:
+        loss, g = tr.gradient(f, y.reshape(B, -1))
+    lt, gt = _torch_grad(flat, f, y)
+    assert abs(loss - lt) <= 2e-2 * abs(lt)
+    off, errs = 0, []
+    for a, b, co, t in unet.SPEC:
+        n = co * t * (a + b) + co
+        ga, gb = g[off:off + n], gt[off:off + n]
+        err = np.linalg.norm(ga - gb) / (np.linalg.norm(gb) + 1e-30)
+        cos = float(ga @ gb / (np.linalg.norm(ga) * np.linalg.norm(gb) + 1e-30))
+        errs.append((round(float(err), 4), round(cos, 5)))
+        off += n
+    print("per-layer relative error, cosine:", errs, flush=True)
+    # BF16 activations and data gradients: the error grows with the depth of
+    # the backward chain (layer 0 is last); a wrong index or sign is O(1).
+    for err, cos in errs:
+        assert err < 0.1 and cos > 0.995
+
+
+@pytest.mark.gpu
+def test_native_training_reduces_loss():
+    """A few Adam steps on a fixed batch lower the loss."""
+    torch = pytest.importorskip("torch")
+    torch.manual_seed(9)
+    flat = unet.flatten(unet.make_model())
+    rng = np.random.default_rng(10)
+    B, Cr = 2, 16
+    f = rng.standard_normal((B, Cr, Cr, Cr, 3)).astype(np.float32)
+    y = (0.5 * f[..., 0] + 0.1).astype(np.float32).reshape(B, -1)
+    with capi.NetTrainer(flat, crop=Cr, batch=B) as tr:
+        l0 = tr.step(f, y, 1e-3)
+        for _ in range(30):
+            l1 = tr.step(f, y, 1e-3)
+        p = tr.params()
+    assert l1 < 0.5 * l0, (l0, l1)
+    assert np.isfinite(p).all() and not np.array_equal(p, flat)
