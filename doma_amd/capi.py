@@ -61,6 +61,7 @@ class Stats(C.Structure):
         ("si_residual", C.c_double * MAX_GROUPS), ("elapsed_ms", C.c_double),
         ("sweep_ms", C.c_double), ("corrections_accepted", C.c_int32),
         ("corrections_rejected", C.c_int32), ("correction_ms", C.c_double),
+        ("k_eff", C.c_double), ("k_residual", C.c_double),
     ]
 
 
@@ -101,6 +102,7 @@ SIGNATURES = [
     ("doma_create", C.c_int, [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("doma_destroy", C.c_int, [C.c_void_p]),
     ("doma_set_materials", C.c_int, [C.c_void_p, _ip, C.c_int32, _dp, _dp, _dp, _u8p]),
+    ("doma_set_fission", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _u8p, _u8p]),
     ("doma_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("doma_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("doma_sweep", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
@@ -109,6 +111,9 @@ SIGNATURES = [
      [C.c_void_p, C.c_int32, _dp, C.c_double, C.c_int32, _dp, _ip, _dp, _ip]),
     ("doma_solve_fixed_source", C.c_int,
      [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32, _dp, C.POINTER(Stats)]),
+    ("doma_solve_eigenvalue", C.c_int,
+     [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_double, _dp, _dp,
+      C.POINTER(Stats)]),
     ("doma_get_flux", C.c_int, [C.c_void_p, _dp]),
     ("doma_time_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _ip]),
     ("doma_profile_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _ip]),
@@ -210,8 +215,26 @@ def gl_quadrature(n: int):
 
 
 @dataclass
+class Fission:
+    """Fission data of the materials (MaterialProps _g_chi_p, _g_prod, _g_n_beta):
+    chi [M, G] prompt spectrum, prod [M, G] nu*sigma_f, prompt [M, G] prompt
+    fraction 1 - sum_d beta (1 without delayed groups), has_chi / has_prod [M]."""
+    chi: np.ndarray
+    prod: np.ndarray
+    prompt: np.ndarray
+    has_chi: np.ndarray
+    has_prod: np.ndarray
+
+    def normalized(self, G: int) -> "Fission":
+        return Fission(_f64(self.chi).reshape(-1, G), _f64(self.prod).reshape(-1, G),
+                       _f64(self.prompt).reshape(-1, G),
+                       np.ascontiguousarray(self.has_chi, dtype=np.uint8).reshape(-1),
+                       np.ascontiguousarray(self.has_prod, dtype=np.uint8).reshape(-1))
+
+
+@dataclass
 class Problem:
-    """A fixed-source problem in the C-ABI layout (numpy, host)."""
+    """A problem in the C-ABI layout (numpy, host); `fission` is optional."""
     dims: int
     lx: np.ndarray
     ly: np.ndarray
@@ -229,6 +252,7 @@ class Problem:
     w: np.ndarray
     name: str = ""
     meta: dict = field(default_factory=dict)
+    fission: Fission | None = None
 
     @property
     def nx(self):
@@ -267,7 +291,8 @@ class Problem:
             _f64(self.sigs).reshape(-1, self.groups, self.groups),
             _f64(self.src).reshape(-1, self.groups),
             np.ascontiguousarray(self.has_src, dtype=np.uint8).reshape(-1),
-            _f64(self.mu), _f64(self.eta), _f64(self.xi), _f64(self.w), self.name, dict(self.meta))
+            _f64(self.mu), _f64(self.eta), _f64(self.xi), _f64(self.w), self.name, dict(self.meta),
+            None if self.fission is None else self.fission.normalized(int(self.groups)))
 
 
 class Context:
@@ -285,6 +310,11 @@ class Context:
         _check(lib().doma_set_materials(h, p.mat.ctypes.data_as(_ip), p.num_mats, _d(p.sigt),
                                         _d(p.sigs), _d(p.src), p.has_src.ctypes.data_as(_u8p)),
                "doma_set_materials")
+        if p.fission is not None:
+            f = p.fission
+            _check(lib().doma_set_fission(h, p.num_mats, _d(f.chi), _d(f.prod), _d(f.prompt),
+                                          f.has_chi.ctypes.data_as(_u8p),
+                                          f.has_prod.ctypes.data_as(_u8p)), "doma_set_fission")
         if tuning:
             self.set_tuning(**tuning)
 
@@ -342,6 +372,19 @@ class Context:
             phi = phi.reshape(self.problem.groups, self.problem.N)
         return phi, stats
 
+    def solve_eigenvalue(self, fs_tol: float, fs_max_it: int, mg_tol: float, mg_max_it: int,
+                         k_tol: float, want_flux: bool = True):
+        """Power iteration: (k, phi[G, N] or None, stats)."""
+        phi = np.zeros(self.problem.groups * self.problem.N) if want_flux else None
+        st, k = Stats(), C.c_double()
+        _check(lib().doma_solve_eigenvalue(self.h, fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol,
+                                           _d(phi) if want_flux else None, C.byref(k),
+                                           C.byref(st)), "doma_solve_eigenvalue")
+        stats = _stats_dict(st, self.problem.groups)
+        if want_flux:
+            phi = phi.reshape(self.problem.groups, self.problem.N)
+        return k.value, phi, stats
+
     def get_flux(self) -> np.ndarray:
         phi = np.zeros(self.problem.groups * self.problem.N)
         _check(lib().doma_get_flux(self.h, _d(phi)), "doma_get_flux")
@@ -392,7 +435,8 @@ def _stats_dict(st: Stats, G: int) -> dict:
             "si_iterations": list(st.si_iterations[:G]), "si_residual": list(st.si_residual[:G]),
             "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms,
             "corrections_accepted": st.corrections_accepted,
-            "corrections_rejected": st.corrections_rejected, "correction_ms": st.correction_ms}
+            "corrections_rejected": st.corrections_rejected, "correction_ms": st.correction_ms,
+            "k_eff": st.k_eff, "k_residual": st.k_residual}
 
 
 def net_param_count() -> int:
@@ -587,9 +631,9 @@ def deck_read(path) -> dict:
 
 def run_deck(path, output_base=None, precision: int = FP64, tuning: dict | None = None,
              autotune: bool = False):
-    """Solve a fixed-source deck on the GPU like the reference's main():
-    returns (converged, phi[G, N], stats dict).  With output_base the
-    reference's text files <base>_*.txt are written."""
+    """Solve a fixed-source or eigenvalue deck on the GPU like the reference's
+    main(): returns (converged, phi[G, N], stats dict; k in stats["k_eff"]).
+    With output_base the reference's text files <base>_*.txt are written."""
     info = deck_read(path)
     N = info["nx"] * info["ny"] * info["nz"]
     phi = np.zeros((info["groups"], N))
@@ -605,5 +649,6 @@ def run_deck(path, output_base=None, precision: int = FP64, tuning: dict | None 
     stats = {"success": st.success, "mg_iterations": st.mg_iterations,
              "mg_residual": st.mg_residual, "sweeps": st.sweeps,
              "si_iterations": list(st.si_iterations[:G]), "si_residual": list(st.si_residual[:G]),
-             "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms}
+             "elapsed_ms": st.elapsed_ms, "sweep_ms": st.sweep_ms, "k_eff": st.k_eff,
+             "k_residual": st.k_residual}
     return bool(conv.value), phi, stats

@@ -106,6 +106,14 @@ iter_args(doma_ctx * c, int g)
   a.red = c->d_red;
   a.res = c->d_res;
   a.err = c->d_err;
+  if (c->fission)
+  {
+    a.chi = c->d_chi;
+    a.prod = c->d_prod;
+    a.prompt = c->d_prompt;
+    a.has_chi = c->d_has_chi;
+    a.has_prod = c->d_has_prod;
+  }
   return a;
 }
 
@@ -714,14 +722,14 @@ create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
       (rc = dev_alloc(&c->d_prev, N * c->G)))
     return cleanup(rc);
   c->red_blocks = reduction_blocks(c->N);
-  if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 4)) ||
+  if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 8)) ||
       (rc = dev_alloc(&c->d_err, 1)) || (rc = dev_alloc(&c->d_ticket, 1)))
     return cleanup(rc);
   if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess ||
       hipMemset(c->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess ||
-      hipMemset(c->d_res, 0, 4 * sizeof(double)) != hipSuccess)
+      hipMemset(c->d_res, 0, 8 * sizeof(double)) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipMemset failed"));
-  if (hipHostMalloc(reinterpret_cast<void **>(&c->h_res), 4 * sizeof(double)) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void **>(&c->h_res), 8 * sizeof(double)) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
   if (hipMemcpy(c->d_lx, lxv.data(), lxv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_ly, lyv.data(), lyv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -780,6 +788,11 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_sigs);
   dev_free(c->d_src);
   dev_free(c->d_has_src);
+  dev_free(c->d_chi);
+  dev_free(c->d_prod);
+  dev_free(c->d_prompt);
+  dev_free(c->d_has_chi);
+  dev_free(c->d_has_prod);
   dev_free(c->d_sig);
   dev_free(c->d_q);
   dev_free(c->d_qmg);
@@ -829,12 +842,18 @@ doma_set_materials(doma_ctx * c, const int32_t * mat, int32_t M, const double * 
   for (size_t e = 0; e < (size_t)M * G * G; ++e)
     if (!std::isfinite(sigs[e]))
       return fail(DOMA_ERR_INVALID, "non-finite scattering cross section");
+  c->fission = false;
   if (c->M != M)
   {
     dev_free(c->d_sigt);
     dev_free(c->d_sigs);
     dev_free(c->d_src);
     dev_free(c->d_has_src);
+    dev_free(c->d_chi);
+    dev_free(c->d_prod);
+    dev_free(c->d_prompt);
+    dev_free(c->d_has_chi);
+    dev_free(c->d_has_prod);
     c->M = 0;
     if ((rc = dev_alloc(&c->d_sigt, (size_t)M * G)) || (rc = dev_alloc(&c->d_sigs, (size_t)M * G * G)) ||
         (rc = dev_alloc(&c->d_src, (size_t)M * G)) || (rc = dev_alloc(&c->d_has_src, (size_t)M)))
@@ -849,6 +868,50 @@ doma_set_materials(doma_ctx * c, const int32_t * mat, int32_t M, const double * 
   DOMA_HIP(hipMemcpy(c->d_src, src, sizeof(double) * M * G, hipMemcpyHostToDevice));
   DOMA_HIP(hipMemcpy(c->d_has_src, has_src, (size_t)M, hipMemcpyHostToDevice));
   c->materials = true;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_fission(doma_ctx * c, int32_t M, const double * chi, const double * prod,
+                 const double * prompt, const uint8_t * has_chi, const uint8_t * has_prod)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!chi || !prod || !prompt || !has_chi || !has_prod)
+    return fail(DOMA_ERR_INVALID, "doma_set_fission: null array");
+  if (M != c->M)
+    return fail(DOMA_ERR_INVALID, "doma_set_fission: " + std::to_string(M) +
+                                      " materials given, doma_set_materials set " +
+                                      std::to_string(c->M));
+  const size_t G = c->G;
+  for (int m = 0; m < M; ++m)
+  {
+    if (has_chi[m] && !has_prod[m])
+      return fail(DOMA_ERR_INVALID, "material " + std::to_string(m) +
+                                        " has a prompt fission spectrum but no nu_sigma_f");
+    for (size_t g = 0; g < G; ++g)
+    {
+      const size_t e = (size_t)m * G + g;
+      if ((has_chi[m] && (!std::isfinite(chi[e]) || !std::isfinite(prompt[e]))) ||
+          (has_prod[m] && !std::isfinite(prod[e])))
+        return fail(DOMA_ERR_INVALID, "non-finite fission data");
+    }
+  }
+  if (!c->d_chi)
+  {
+    if ((rc = dev_alloc(&c->d_chi, (size_t)M * G)) || (rc = dev_alloc(&c->d_prod, (size_t)M * G)) ||
+        (rc = dev_alloc(&c->d_prompt, (size_t)M * G)) || (rc = dev_alloc(&c->d_has_chi, (size_t)M)) ||
+        (rc = dev_alloc(&c->d_has_prod, (size_t)M)))
+      return rc;
+  }
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipMemcpy(c->d_chi, chi, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_prod, prod, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_prompt, prompt, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_has_chi, has_chi, (size_t)M, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_has_prod, has_prod, (size_t)M, hipMemcpyHostToDevice));
+  c->fission = true;
   return DOMA_OK;
 }
 
@@ -1022,6 +1085,90 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
   DOMA_HIP(hipStreamSynchronize(c->stream));
   DOMA_HIP(hipGetLastError());
   s.elapsed_ms = now_ms() - t0;
+  if (st)
+    *st = s;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_solve_eigenvalue(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_tol,
+                      int32_t mg_max_it, double k_tol, double * phi, double * k_eff,
+                      doma_stats * st)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!c->fission)
+    return fail(DOMA_ERR_STATE, "an eigenvalue solve needs fission data (doma_set_fission)");
+  if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || !(k_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
+    return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
+  const double t0 = now_ms();
+  doma_stats s{};
+  const size_t N = (size_t)c->N;
+  // initializeSolve, RunMode::Eigen (TransportSolver3D.cpp:499-529): phi = 1,
+  // previous iterate and iteration source 0; k = 1, k_prev = 0 (constructor).
+  if ((rc = launch_fill(c->d_phi, N * c->G, 1.0, c->stream)) ||
+      (rc = launch_fill(c->d_prev, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_q, N, 0.0, c->stream)))
+    return rc;
+  double k = 1.0, k_prev = 0.0, shape = 0.0, kdiff = 0.0;
+  int mg_it = 0, ok = 1;
+  do
+  {
+    for (int g = 0; g < c->G; ++g)
+    {
+      IterArgs a = iter_args(c, g);
+      a.eigen = 1;
+      a.k = k;
+      if ((rc = launch_mg_source(a, c->stream)))
+        return rc;
+      const bool warm = c->warm_start && mg_it > 0 && c->dims > 1;
+      if (warm && (rc = launch_warm_start(a, c->stream)))
+        return rc;
+      int its = 0, conv = 0;
+      double res = 0.0;
+      if ((rc = si_loop(c, g, fs_tol, fs_max_it, &its, &res, &conv, &s.sweep_ms, &s.sweeps, &s,
+                        warm ? c->d_prev + (size_t)g * c->N : nullptr)))
+        return rc;
+      s.si_iterations[g] = its;
+      s.si_residual[g] = res;
+      if (!conv)
+      {
+        ok = 0;
+        break;
+      }
+    }
+    if (!ok)
+      break;
+    // updateEigenvalue, then computeMGFluxResidual (TransportSolver3D.cpp:59-62).
+    IterArgs a = iter_args(c, 0);
+    a.eigen = 1;
+    a.k = k;
+    if ((rc = launch_fission_rates(a, c->red_blocks, c->stream)) ||
+        (rc = launch_mg_residual(a, c->red_blocks, c->stream)))
+      return rc;
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    DOMA_HIP(hipStreamSynchronize(c->stream));
+    k_prev = k;
+    k = c->dims == 1 ? k_prev * c->h_res[4] / c->h_res[5] : c->h_res[4] / c->h_res[5];
+    shape = c->h_res[0];
+    kdiff = std::fabs(k - k_prev);
+    ++mg_it;
+  } while (mg_it < mg_max_it && (mg_tol < shape || k_tol < kdiff));
+  s.success = ok && mg_it < mg_max_it;
+  s.mg_iterations = mg_it;
+  s.mg_residual = shape;
+  s.k_eff = k;
+  s.k_residual = kdiff;
+  if (phi)
+    DOMA_HIP(hipMemcpyAsync(phi, c->d_phi, sizeof(double) * N * c->G, hipMemcpyDeviceToHost,
+                            c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipGetLastError());
+  s.elapsed_ms = now_ms() - t0;
+  if (k_eff)
+    *k_eff = k;
   if (st)
     *st = s;
   return DOMA_OK;

@@ -145,6 +145,13 @@ struct IterArgs
   const double * corr;     // [N] correction applied before this sweep (or null)
   double * dcur;           // [N] receives this iteration's increment (or null)
   const double * dlast;    // [N] last accepted increment (correction kernels)
+  const double * chi;      // [M][G] prompt fission spectrum (null: no fission data)
+  const double * prod;     // [M][G] nu sigma_f
+  const double * prompt;   // [M][G] prompt fraction 1 - sum_d beta
+  const uint8_t * has_chi; // [M]
+  const uint8_t * has_prod;// [M]
+  int eigen;               // 1: power-iteration source (fission / k, no external source)
+  double k;                // current eigenvalue (eigen != 0)
 };
 
 int launch_mg_source(const IterArgs & a, hipStream_t s);
@@ -153,6 +160,9 @@ int launch_finish(const IterArgs & a, int nblocks, hipStream_t s);
 int launch_residual_final(const IterArgs & a, int nblocks, hipStream_t s);
 int launch_reduce_parts(const IterArgs & a, hipStream_t s);
 int launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s);
+// Fission production sums of updateEigenvalue: res[4] = sum V nuSf phi,
+// res[5] = sum V nuSf prev (/ k in 2D/3D).  Uses the reduction partials.
+int launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s);
 int launch_fill(double * p, size_t n, double v, hipStream_t s);
 // Batched finish/residual over problem slots (see iterate.hip), training-data
 // snapshots and targets.
@@ -199,6 +209,12 @@ struct doma_ctx
   double * d_sigs = nullptr;
   double * d_src = nullptr;
   uint8_t * d_has_src = nullptr;
+  bool fission = false;  // doma_set_fission data present for the current materials
+  double * d_chi = nullptr;
+  double * d_prod = nullptr;
+  double * d_prompt = nullptr;
+  uint8_t * d_has_chi = nullptr;
+  uint8_t * d_has_prod = nullptr;
 
   // iteration state (device)
   double * d_sig = nullptr;

@@ -78,9 +78,29 @@ block_sum2(double & u, double & v)
   v = sv[0];
 }
 
+// Prompt fission term of group g from group g' (TransportSolver3D.cpp:377-387,
+// TransportSolver2D.cpp:377-387, TransportSolver1D.cpp:410-420):
+// norm * f * chi_g * nuSf_g' * phi_g' (/ pi in 2D/3D), f the prompt fraction
+// (1 - sum_d beta, exactly 1 without delayed groups).
+__device__ __forceinline__ double
+fission_term(int dims, double f, double chi, double prod, double phi)
+{
+#pragma clang fp contract(off)
+  double t = (dims == 3 ? 0.25 : 0.5) * f;
+  t = t * chi;
+  t = t * prod;
+  t = t * phi;
+  return dims == 1 ? t : t / PI;
+}
+
 // updateMultigroupSource (TransportSolver3D.cpp:339-402; 2D :339-402; 1D
-// TransportSolver1D.cpp:369-435), fixed-source terms, fused with the gather of
-// the group's total cross section.
+// TransportSolver1D.cpp:369-435), steady-state terms (external source,
+// in-scatter, prompt fission), fused with the gather of the group's total cross
+// section.  With a.eigen, updateMultigroupSourceEigen (TransportSolver3D.cpp:
+// 404-430; 2D :404-430; 1D TransportSolver1D.cpp:437-466): in-scatter plus
+// fission / k and no external source.  In 2D/3D the reference does not clear the
+// iteration source first, so the previous group's last scattering update (left
+// in q by the source iteration) stays in it; 1D starts from zero.
 __global__ void
 mg_source_kernel(IterArgs a)
 {
@@ -90,10 +110,27 @@ mg_source_kernel(IterArgs a)
     return;
   const int m = a.mat[c];
   const int G = a.G, g = a.g;
-  double acc = a.has_src[m] ? norm_src(a.dims, a.src[(size_t)m * G + g]) : 0.0;
+  const bool fis = a.chi && a.has_chi[m];
+  double acc;
+  if (a.eigen)
+    acc = a.dims == 1 ? 0.0 : a.q[c];
+  else
+    acc = a.has_src[m] ? norm_src(a.dims, a.src[(size_t)m * G + g]) : 0.0;
+  const double chi = fis ? a.chi[(size_t)m * G + g] : 0.0;
   for (int gp = 0; gp < G; ++gp)
+  {
+    const double f = a.phi[(size_t)gp * a.N + c];
     if (gp != g)
-      acc += norm_term(a.dims, a.sigs[((size_t)m * G + g) * G + gp], a.phi[(size_t)gp * a.N + c]);
+      acc += norm_term(a.dims, a.sigs[((size_t)m * G + g) * G + gp], f);
+    if (fis)
+    {
+      const double pr = a.prod[(size_t)m * G + gp];
+      if (a.eigen)
+        acc += fission_term(a.dims, 1.0, chi, pr, f) / a.k;
+      else
+        acc += fission_term(a.dims, a.prompt[(size_t)m * G + gp], chi, pr, f);
+    }
+  }
   if (a.dims == 1)
     a.qmg[c] = acc;
   else
@@ -320,6 +357,56 @@ mg_residual_kernel(IterArgs a)
   }
 }
 
+// updateEigenvalue sums (TransportSolver3D.cpp:432-456; 2D :432-456; 1D
+// TransportSolver1D.cpp:468-493) over materials with nu_sigma_f: per cell and
+// group, num += V nuSf phi and den += V nuSf prev (/ k in 2D/3D).
+__global__ __launch_bounds__(RED_THREADS) void
+fission_rate_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  double num = 0.0, den = 0.0;
+  const int stride = gridDim.x * RED_THREADS;
+  for (int c = blockIdx.x * RED_THREADS + threadIdx.x; c < a.N; c += stride)
+  {
+    const int m = a.mat[c];
+    if (!a.has_prod[m])
+      continue;
+    const double v = measure(a, c);
+    for (int g = 0; g < a.G; ++g)
+    {
+      const double vp = v * a.prod[(size_t)m * a.G + g];
+      num = num + vp * a.phi[(size_t)g * a.N + c];
+      if (a.dims == 1)
+        den = den + vp * a.prev[(size_t)g * a.N + c];
+      else
+        den = den + vp * a.prev[(size_t)g * a.N + c] / a.k;
+    }
+  }
+  block_sum2(num, den);
+  if (threadIdx.x == 0)
+  {
+    a.red[2 * blockIdx.x + 0] = num;
+    a.red[2 * blockIdx.x + 1] = den;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void
+rate_final_kernel(IterArgs a, int nblocks)
+{
+  double num = 0.0, den = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += RED_THREADS)
+  {
+    num += a.red[2 * b + 0];
+    den += a.red[2 * b + 1];
+  }
+  block_sum2(num, den);
+  if (threadIdx.x == 0)
+  {
+    a.res[4] = num;
+    a.res[5] = den;
+  }
+}
+
 __global__ void
 fill_kernel(double * p, size_t n, double v)
 {
@@ -435,6 +522,14 @@ launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s)
 {
   hipLaunchKernelGGL(mg_residual_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
   hipLaunchKernelGGL(residual_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(fission_rate_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
+  hipLaunchKernelGGL(rate_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
   return DOMA_OK;
 }
 

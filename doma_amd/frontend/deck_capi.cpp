@@ -80,7 +80,7 @@ namespace
 {
 template <typename Solver, typename Mesh>
 int
-run_solver(Solver & s, Mesh & m, const char * output_base, int32_t precision,
+run_solver(Solver & s, Mesh & m, RunMode mode, const char * output_base, int32_t precision,
            const doma_tuning * tuning, int autotune, double * phi, int64_t phi_len, doma_stats * st,
            int32_t * converged)
 {
@@ -88,7 +88,8 @@ run_solver(Solver & s, Mesh & m, const char * output_base, int32_t precision,
   if (tuning)
     s.options().tuning = *tuning;
   s.options().autotune = autotune != 0;
-  const bool ok = s.solveFixedSource(output_base ? output_base : "");
+  const std::string base = output_base ? output_base : "";
+  const bool ok = mode == RunMode::Eigen ? s.solveEigenvalue(base) : s.solveFixedSource(base);
   if (!s.lastError().empty())
     return fail(DOMA_ERR_INVALID, s.lastError());
   if (converged)
@@ -115,8 +116,8 @@ doma_run_deck(const char * path, const char * output_base, int32_t precision,
   try
   {
     const InputParameters p = parseInputParameters(path);
-    if (p._mode != RunMode::FixedSrc)
-      return fail(DOMA_ERR_INVALID, "only fixed-source decks (solve_type=\"steady\") are supported");
+    if (p._mode == RunMode::Transient)
+      return fail(DOMA_ERR_INVALID, "transient decks (solve_type=\"transient\") are not supported");
     if (p._num_dims == 1)
     {
       if (p._eq_type == EquationType::TW_DD)
@@ -125,7 +126,7 @@ doma_run_deck(const char * path, const char * output_base, int32_t precision,
       BrickMesh1D m(p, {BoundaryCondition::Vacuum, BoundaryCondition::Vacuum}, 1);
       m.validateProps();
       DDTransportSolver1D s(m, p, false, p._dsa, 1);
-      return run_solver(s, m, output_base, precision, tuning, autotune, phi, phi_len, stats,
+      return run_solver(s, m, p._mode, output_base, precision, tuning, autotune, phi, phi_len, stats,
                         converged);
     }
     if (p._num_dims == 2)
@@ -137,11 +138,11 @@ doma_run_deck(const char * path, const char * output_base, int32_t precision,
       if (p._eq_type == EquationType::TW_DD)
       {
         TWDDTransportSolver2D s(m, p, false, 1);
-        return run_solver(s, m, output_base, precision, tuning, autotune, phi, phi_len, stats,
+        return run_solver(s, m, p._mode, output_base, precision, tuning, autotune, phi, phi_len, stats,
                           converged);
       }
       DDTransportSolver2D s(m, p, false, 1);
-      return run_solver(s, m, output_base, precision, tuning, autotune, phi, phi_len, stats,
+      return run_solver(s, m, p._mode, output_base, precision, tuning, autotune, phi, phi_len, stats,
                         converged);
     }
     if (p._num_dims == 3)
@@ -153,11 +154,11 @@ doma_run_deck(const char * path, const char * output_base, int32_t precision,
       if (p._eq_type == EquationType::TW_DD)
       {
         TWDDTransportSolver3D s(m, p, false, 1);
-        return run_solver(s, m, output_base, precision, tuning, autotune, phi, phi_len, stats,
+        return run_solver(s, m, p._mode, output_base, precision, tuning, autotune, phi, phi_len, stats,
                           converged);
       }
       DDTransportSolver3D s(m, p, false, 1);
-      return run_solver(s, m, output_base, precision, tuning, autotune, phi, phi_len, stats,
+      return run_solver(s, m, p._mode, output_base, precision, tuning, autotune, phi, phi_len, stats,
                         converged);
     }
     return fail(DOMA_ERR_INVALID, "mesh dims must be 1, 2 or 3");

@@ -1,6 +1,7 @@
 // doma_main.cpp -- command-line driver with the reference's interface
 // (src/main.cpp:16-298): doma -i deck.xml [--verbose] [-n threads], solving
-// fixed-source decks on the GPU and writing <deck dir>/<deck stem>_*.txt.
+// fixed-source and eigenvalue decks on the GPU and writing
+// <deck dir>/<deck stem>_*.txt.
 // Extra options of this build: --fp32 (FP32-sweep mode), --autotune.
 #include <chrono>
 #include <cstring>
@@ -21,16 +22,17 @@ usage(const char * argv0)
 {
   std::cerr << "Usage: " << argv0
             << " -i DECK.xml [--verbose] [-n N_THREADS] [--fp32] [--autotune]\n"
-               "Discrete Ordinates Mini-App (MI355X build): fixed-source S_N solves on the GPU.\n";
+               "Discrete Ordinates Mini-App (MI355X build): fixed-source and k-eigenvalue S_N "
+               "solves on the GPU.\n";
 }
 
 template <typename S>
 bool
-solve(S & s, bool fp32, bool autotune, const std::string & base)
+solve(S & s, RunMode mode, bool fp32, bool autotune, const std::string & base)
 {
   s.options().precision = fp32 ? DOMA_FP32_SWEEP : DOMA_FP64;
   s.options().autotune = autotune;
-  return s.solveFixedSource(base);
+  return mode == RunMode::Eigen ? s.solveEigenvalue(base) : s.solveFixedSource(base);
 }
 } // namespace
 
@@ -79,9 +81,10 @@ main(int argc, char ** argv)
   try
   {
     const InputParameters params = parseInputParameters(inp.string());
-    if (params._mode != RunMode::FixedSrc)
+    if (params._mode == RunMode::Transient)
     {
-      std::cerr << "This build solves fixed-source problems only (solve_type=\"steady\")."
+      std::cerr << "Transient mode is outside this build's scope (solve_type=\"steady\" or "
+                   "\"eigen\")."
                 << std::endl;
       return 1;
     }
@@ -98,7 +101,7 @@ main(int argc, char ** argv)
       BrickMesh1D mesh(params, {BoundaryCondition::Vacuum, BoundaryCondition::Vacuum}, threads);
       mesh.validateProps();
       DDTransportSolver1D s(mesh, params, verbose, params._dsa, threads);
-      ok = solve(s, fp32, autotune, base);
+      ok = solve(s, params._mode, fp32, autotune, base);
     }
     else if (params._num_dims == 2)
     {
@@ -109,12 +112,12 @@ main(int argc, char ** argv)
       if (params._eq_type == EquationType::DD)
       {
         DDTransportSolver2D s(mesh, params, verbose, threads);
-        ok = solve(s, fp32, autotune, base);
+        ok = solve(s, params._mode, fp32, autotune, base);
       }
       else
       {
         TWDDTransportSolver2D s(mesh, params, verbose, threads);
-        ok = solve(s, fp32, autotune, base);
+        ok = solve(s, params._mode, fp32, autotune, base);
       }
     }
     else if (params._num_dims == 3)
@@ -126,12 +129,12 @@ main(int argc, char ** argv)
       if (params._eq_type == EquationType::DD)
       {
         DDTransportSolver3D s(mesh, params, verbose, threads);
-        ok = solve(s, fp32, autotune, base);
+        ok = solve(s, params._mode, fp32, autotune, base);
       }
       else
       {
         TWDDTransportSolver3D s(mesh, params, verbose, threads);
-        ok = solve(s, fp32, autotune, base);
+        ok = solve(s, params._mode, fp32, autotune, base);
       }
     }
     else
@@ -141,7 +144,9 @@ main(int argc, char ** argv)
     }
     if (!ok)
       return 2;
-    std::cout << "Fixed source solver finished executing." << std::endl;
+    std::cout << (params._mode == RunMode::Eigen ? "Eigenvalue solver finished executing."
+                                                 : "Fixed source solver finished executing.")
+              << std::endl;
     const double s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     std::cout << "Time elapsed: " << s << " (s)." << std::endl;

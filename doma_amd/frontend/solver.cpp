@@ -36,7 +36,7 @@ SolverCore::fixedSourceOnly()
 bool
 SolverCore::run(const std::string & output_file_base, int dims, int dirs,
                 const std::vector<double> & mu, const std::vector<double> & eta,
-                const std::vector<double> & xi, const std::vector<double> & w)
+                const std::vector<double> & xi, const std::vector<double> & w, bool eigen)
 {
   _error.clear();
   const unsigned int G = _params._num_e_groups;
@@ -56,10 +56,49 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
   const int M = m;
   std::vector<double> sigt((size_t)M * G), sigs((size_t)M * G * G), src((size_t)M * G, 0.0);
   std::vector<uint8_t> has(M, 0);
+  // Fission data: chi_p, nu_sigma_f and the prompt fraction 1 - sum_d beta of a
+  // material with delayed groups (TransportSolver3D.cpp:377-387).
+  std::vector<double> chi((size_t)M * G, 0.0), prod((size_t)M * G, 0.0), prompt((size_t)M * G, 1.0);
+  std::vector<uint8_t> has_chi(M, 0), has_prod(M, 0);
+  bool fission = false;
   for (const auto & kv : index)
   {
     const MaterialProps & p = _params._block_mat_info.at(kv.first);
     const int mi = kv.second;
+    if (!p._g_chi_p.empty() && p._g_prod.empty())
+    {
+      _error = "Block " + std::to_string(kv.first) +
+               " has a prompt fission spectrum (chi_p) but no nu_sigma_f!";
+      std::cerr << _error << std::endl;
+      return false;
+    }
+    if (!p._g_chi_p.empty() && p._num_d_groups > 0 &&
+        p._g_n_beta.size() != (size_t)G * p._num_d_groups)
+    {
+      _error = "Not enough delayed fractions have been provided for block " +
+               std::to_string(kv.first) + "!";
+      std::cerr << _error << std::endl;
+      return false;
+    }
+    if (!p._g_chi_p.empty())
+    {
+      has_chi[mi] = 1;
+      for (unsigned int g = 0; g < G; ++g)
+      {
+        chi[(size_t)mi * G + g] = p._g_chi_p[g];
+        double gb = 0.0;
+        for (unsigned int d = 0; d < p._num_d_groups; ++d)
+          gb += p._g_n_beta[g * p._num_d_groups + d];
+        prompt[(size_t)mi * G + g] = 1.0 - gb;
+      }
+    }
+    if (!p._g_prod.empty())
+    {
+      has_prod[mi] = 1;
+      for (unsigned int g = 0; g < G; ++g)
+        prod[(size_t)mi * G + g] = p._g_prod[g];
+    }
+    fission = fission || has_chi[mi] || has_prod[mi];
     for (unsigned int g = 0; g < G; ++g)
       sigt[(size_t)mi * G + g] = p._g_total[g];
     for (unsigned int e = 0; e < G * G; ++e)
@@ -109,13 +148,20 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
   int rc = doma_create(&d, &ctx);
   if (rc == DOMA_OK)
     rc = doma_set_materials(ctx, mat.data(), M, sigt.data(), sigs.data(), src.data(), has.data());
+  if (rc == DOMA_OK && fission)
+    rc = doma_set_fission(ctx, M, chi.data(), prod.data(), prompt.data(), has_chi.data(),
+                          has_prod.data());
   if (rc == DOMA_OK && (_opt.tuning.brick || _opt.tuning.threads || _opt.tuning.exact_arith ||
                         _opt.tuning.schedule || _opt.tuning.octants || _opt.tuning.dirs_per_lane ||
                         _opt.tuning.shape))
     rc = doma_set_tuning(ctx, &_opt.tuning);
   if (rc == DOMA_OK && _opt.autotune)
     rc = doma_autotune(ctx, 0, nullptr, nullptr);
-  if (rc == DOMA_OK)
+  if (rc == DOMA_OK && eigen)
+    rc = doma_solve_eigenvalue(ctx, _params._src_it_tol, (int)_params._num_src_it, _params._gs_tol,
+                               (int)_params._num_mg_it, _params._k_tol, _mesh.flux().data(),
+                               nullptr, &_stats);
+  else if (rc == DOMA_OK)
     rc = doma_solve_fixed_source(ctx, _params._src_it_tol, (int)_params._num_src_it, _params._gs_tol,
                                  (int)_params._num_mg_it, _mesh.flux().data(), &_stats);
   if (rc != DOMA_OK)
@@ -136,7 +182,24 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
               << _stats.si_iterations[g] << " iterations with a residual of "
               << _stats.si_residual[g] << " for G" << g << std::endl;
   }
-  if (_stats.success)
+  if (eigen)
+  {
+    // solveEigenvalue's summary (TransportSolver3D.cpp:65-85).
+    if (_stats.success)
+    {
+      std::cout << "PI converged after " << _stats.mg_iterations
+                << " iterations with a flux shape residual of " << _stats.mg_residual
+                << " and a k residual of " << _stats.k_residual << "." << std::endl
+                << "Converged value of k = " << _stats.k_eff << std::endl;
+      if (!output_file_base.empty())
+        _mesh.dumpToTextFile(output_file_base);
+    }
+    else if (_stats.mg_iterations >= (int)_params._num_mg_it)
+      std::cout << "PI failed to converge after " << _stats.mg_iterations
+                << " iterations with a flux shape residual of " << _stats.mg_residual
+                << " and a k residual of " << _stats.k_residual << "." << std::endl;
+  }
+  else if (_stats.success)
   {
     if (G > 1)
       std::cout << "MGI converged after " << _stats.mg_iterations << " iterations with a residual of "
@@ -203,9 +266,12 @@ TransportSolver3D<T>::solveFixedSource(const std::string & output_file_base, con
 
 template <typename T>
 bool
-TransportSolver3D<T>::solveEigenvalue(const std::string &)
+TransportSolver3D<T>::solveEigenvalue(const std::string & output_file_base)
 {
-  return refuse("Eigenvalue mode is outside this build's scope.");
+  _error.clear();
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 3, mu, eta, xi, w);
+  return run(output_file_base, 3, per, mu, eta, xi, w, true);
 }
 
 template <typename T>
@@ -238,9 +304,12 @@ TransportSolver2D<T>::solveFixedSource(const std::string & output_file_base, con
 
 template <typename T>
 bool
-TransportSolver2D<T>::solveEigenvalue(const std::string &)
+TransportSolver2D<T>::solveEigenvalue(const std::string & output_file_base)
 {
-  return refuse("Eigenvalue mode is outside this build's scope.");
+  _error.clear();
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 2, mu, eta, xi, w);
+  return run(output_file_base, 2, per, mu, eta, xi, w, true);
 }
 
 template <typename T>
@@ -280,9 +349,20 @@ TransportSolver1D<T>::solveFixedSource(const std::string & output_file_base, con
 
 template <typename T>
 bool
-TransportSolver1D<T>::solveEigenvalue(const std::string &)
+TransportSolver1D<T>::solveEigenvalue(const std::string & output_file_base)
 {
-  return refuse("Eigenvalue mode is outside this build's scope.");
+  _error.clear();
+  if (_dsa)
+    return refuse("Diffusion synthetic acceleration is outside this build's scope; solve with "
+                  "dsa=\"false\".");
+  const int n = (int)_angular_quad.order();
+  std::vector<double> mu(n), eta(n, 0.0), xi(n, 0.0), w(n);
+  for (int i = 0; i < n; ++i)
+  {
+    mu[i] = _angular_quad.direction(i);
+    w[i] = _angular_quad.weight(i);
+  }
+  return run(output_file_base, 1, n / 2, mu, eta, xi, w, true);
 }
 
 template <typename T>

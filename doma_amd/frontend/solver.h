@@ -40,9 +40,10 @@ public:
 
 protected:
   SolverCore(BrickMesh & mesh, const InputParameters & params, bool verbose, int scheme);
+  // eigen = false: solveFixedSource; true: solveEigenvalue (power iteration).
   bool run(const std::string & output_file_base, int dims, int dirs, const std::vector<double> & mu,
            const std::vector<double> & eta, const std::vector<double> & xi,
-           const std::vector<double> & w);
+           const std::vector<double> & w, bool eigen = false);
   bool fixedSourceOnly();
   bool refuse(const std::string & why);
 
@@ -63,8 +64,9 @@ public:
                     unsigned int num_threads);
   // Solve the fixed-source problem (TransportSolver3D.cpp:262-316).
   bool solveFixedSource(const std::string & output_file_base = "", const double & t = 0.0);
-  // Eigenvalue and transient modes are outside this build's scope.
+  // k-eigenvalue power iteration (TransportSolver3D.cpp:29-86).
   bool solveEigenvalue(const std::string & output_file_base = "");
+  // Transient mode is outside this build's scope.
   bool solveTransient(const std::string & output_file_base = "");
   const GCAngularQuadrature & quadrature() const { return _angular_quad; }
 

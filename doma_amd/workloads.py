@@ -18,6 +18,10 @@ seeded-media generator (no oracle involved).
   log-uniform in [0.5, 10].
 * ``batched_uneven`` -- independent seeded problems of uneven size for the
   batched path.
+* ``fissile_blocks`` -- ``random_blocks`` with fission data on a seeded share of
+  the blocks: chi_p concentrated in the fastest groups, nu*sigma_f a seeded
+  multiple of each group's absorption; below critical for fixed-source solves
+  (``nu_ratio`` < 1), near critical for power iteration.
 """
 from __future__ import annotations
 
@@ -130,3 +134,55 @@ def batched_uneven(count: int = 512, seed: int = SEED_BASE + 3, np_: int = 4, na
         out.append(block_media_problem(n, g, np_, na, capi.SCHEME_TW_DD, spec,
                                        f"batched_{b}_{n}c_{g}g"))
     return out
+
+
+def fission_for(p: Problem, seed: int, fissile_fraction: float = 0.5,
+                nu_ratio=(0.4, 0.8)) -> capi.Fission:
+    """Fission data for the materials of `p`: a seeded `fissile_fraction` of them
+    (always including those of the first and the middle cell) get chi_p (0.75 / 0.25 over the two fastest groups; all in group 0 with one
+    group) and nu*sigma_f = r * absorption with r ~ U[nu_ratio]; the others have
+    neither.  No delayed groups (prompt fraction 1)."""
+    rng = np.random.default_rng(seed)
+    M, G = p.num_mats, p.groups
+    sigt = np.asarray(p.sigt, dtype=np.float64).reshape(M, G)
+    sigs = np.asarray(p.sigs, dtype=np.float64).reshape(M, G, G)
+    absorb = np.maximum(sigt - sigs.sum(axis=1), 0.05 * sigt)  # out of each source group
+    fis = rng.random(M) < fissile_fraction
+    mat = np.asarray(p.mat).reshape(-1)
+    fis[mat[0]] = fis[mat[mat.size // 2]] = True  # corner and centre cells are fissile
+    chi = np.zeros((M, G))
+    chi[:, 0] = 1.0 if G == 1 else 0.75
+    if G > 1:
+        chi[:, 1] = 0.25
+    chi[~fis] = 0.0
+    prod = rng.uniform(nu_ratio[0], nu_ratio[1], (M, G)) * absorb
+    prod[~fis] = 0.0
+    return capi.Fission(chi, prod, np.ones((M, G)), fis.astype(np.uint8), fis.astype(np.uint8))
+
+
+def fissile_blocks(n: int = 32, groups: int = 3, np_: int = 2, na: int = 2,
+                   scheme: int = capi.SCHEME_TW_DD, seed: int = SEED_BASE + 6,
+                   nu_ratio=(0.4, 0.8), dims: int = 3) -> Problem:
+    """dims 3: an n^3 ``random_blocks`` cube; dims 2: its k = 0 plane as an n x n
+    square with the 2D product quadrature; dims 1: its first row as an n-cell slab
+    with Gauss-Legendre S_(2 np_) (diamond difference)."""
+    block = max(b for b in range(1, 9) if n % b == 0)
+    p = random_blocks(n, groups, np_, na, scheme, seed, block=block)
+    if dims < 3:
+        mat = np.asarray(p.mat).reshape(n, n, n)[0]
+        if dims == 2:
+            mu, eta, xi, w = quadrature_2d(np_, na)
+            p = Problem(2, p.lx, p.ly, np.ones(1), groups, scheme, mat.reshape(-1), p.sigt,
+                        p.sigs, p.src, p.has_src, mu, eta, xi, w, name=p.name, meta=p.meta)
+        else:
+            mu, eta, xi, w = quadrature_1d(np_)
+            p = Problem(1, p.lx, np.ones(1), np.ones(1), groups, capi.SCHEME_DD,
+                        mat[n // 2].copy(), p.sigt, p.sigs, p.src, p.has_src, mu, eta, xi, w,
+                        name=p.name, meta=p.meta)
+        # the central source lies off the slice: put a unit source in every material
+        p.src = np.zeros((p.num_mats, groups))
+        p.src[:, 0] = 1.0
+        p.has_src = np.ones(p.num_mats, dtype=np.uint8)
+    p.fission = fission_for(p, seed, nu_ratio=nu_ratio)
+    p.name = f"fissile_blocks_{dims}d_{n}c_{groups}g"
+    return p

@@ -18,6 +18,8 @@
  *                                                       -> doma_source_iteration
  *   - TransportSolver3D<T>::solveFixedSource (include/TransportSolver3D.h:28;
  *     src/TransportSolver3D.cpp:262-402, 499-529)       -> doma_solve_fixed_source
+ *   - TransportSolver3D<T>::solveEigenvalue (src/TransportSolver3D.cpp:29-86,
+ *     404-456)                                          -> doma_solve_eigenvalue
  *   - the 2D and 1D counterparts (src/TransportSolver2D.cpp:262-725,
  *     src/TransportSolver1D.cpp:292-586, 802-915)       -> same entry points, dims 2/1
  */
@@ -93,6 +95,8 @@ typedef struct doma_stats {
   int32_t corrections_accepted;  /* learned corrections kept (doma_set_learned_correction) */
   int32_t corrections_rejected;  /* learned corrections undone because the residual rose */
   double correction_ms;          /* wall time spent evaluating the correction network */
+  double k_eff;                  /* doma_solve_eigenvalue: final eigenvalue (else 0) */
+  double k_residual;             /* doma_solve_eigenvalue: |k - k_prev| of the last pass */
 } doma_stats;
 
 const char *doma_last_error(void);
@@ -108,6 +112,18 @@ int doma_destroy(doma_ctx *ctx);
  * for a material without an external source (reference: empty _g_src). */
 int doma_set_materials(doma_ctx *ctx, const int32_t *mat, int32_t num_mats, const double *sigt,
                        const double *sigs, const double *src, const uint8_t *has_src);
+
+/* Fission data of the materials set by doma_set_materials (MaterialProps
+ * _g_chi_p, _g_prod, _g_n_beta, include/InputParameters.h:24-55): chi_p[M][G]
+ * prompt fission spectrum, prod[M][G] nu*sigma_f, prompt[M][G] the prompt
+ * fraction 1 - sum_d beta[g][d] of a material with delayed groups (1 without),
+ * has_chi[M] / has_prod[M] = 0 where the reference's vector is empty.  A
+ * material with chi_p must have nu_sigma_f.  Fixed-source solves add the prompt
+ * fission source to the multigroup source (TransportSolver3D.cpp:377-387,
+ * TransportSolver2D.cpp:377-387, TransportSolver1D.cpp:410-420);
+ * doma_solve_eigenvalue requires this call.  doma_set_materials clears it. */
+int doma_set_fission(doma_ctx *ctx, int32_t num_mats, const double *chi_p, const double *prod,
+                     const double *prompt, const uint8_t *has_chi, const uint8_t *has_prod);
 
 int doma_set_tuning(doma_ctx *ctx, const doma_tuning *t);
 int doma_get_tuning(const doma_ctx *ctx, doma_tuning *t);
@@ -132,6 +148,19 @@ int doma_source_iteration(doma_ctx *ctx, int32_t g, const double *qmg, double fs
  * be NULL; the flux stays resident and can be read with doma_get_flux. */
 int doma_solve_fixed_source(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, double mg_tol,
                             int32_t mg_max_it, double *phi, doma_stats *stats);
+
+/* k-eigenvalue power iteration (TransportSolver3D.cpp:29-86, 404-456;
+ * TransportSolver2D.cpp:29-86, 404-456; TransportSolver1D.cpp:36-90, 437-493):
+ * phi = 1, k = 1; each pass forms every group's in-scatter + fission/k source
+ * and runs its source iteration (Gauss-Seidel over groups), then updates
+ * k = sum(V nuSf phi) / sum(V nuSf phi_prev / k) (1D: k_prev * sum(l nuSf phi) /
+ * sum(l nuSf phi_prev)); stops after mg_max_it passes or once the flux shape
+ * residual <= mg_tol and |k - k_prev| <= k_tol.  A source iteration that does
+ * not converge ends the solve unsuccessfully, as in the reference.  The
+ * eigenvalue is returned in *k_eff (may be NULL) and stats->k_eff. */
+int doma_solve_eigenvalue(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, double mg_tol,
+                          int32_t mg_max_it, double k_tol, double *phi, double *k_eff,
+                          doma_stats *stats);
 
 int doma_get_flux(doma_ctx *ctx, double *phi);
 
@@ -261,12 +290,14 @@ typedef struct doma_deck_info {
  * with the reference's message in doma_last_error(). */
 int doma_deck_read(const char *path, doma_deck_info *out);
 
-/* Run a fixed-source deck end to end like the reference's main()
- * (src/main.cpp:51-298): mesh, solver by dimension and scheme, GPU solve; with a
- * non-NULL output_base the reference's text files <base>_dims.txt, _blocks.txt,
- * _meshx/y/z.txt, _g<g>_flux.txt are written.  phi (may be NULL) receives [G][N]
- * of capacity phi_len doubles; *converged is the reference's solveFixedSource
- * result.  tuning may be NULL; autotune != 0 runs doma_autotune first. */
+/* Run a fixed-source (solve_type="steady") or eigenvalue (solve_type="eigen")
+ * deck end to end like the reference's main() (src/main.cpp:51-298): mesh,
+ * solver by dimension and scheme, GPU solve; with a non-NULL output_base the
+ * reference's text files <base>_dims.txt, _blocks.txt, _meshx/y/z.txt,
+ * _g<g>_flux.txt are written.  phi (may be NULL) receives [G][N] of capacity
+ * phi_len doubles; *converged is the reference's solveFixedSource /
+ * solveEigenvalue result and stats->k_eff the eigenvalue.  tuning may be NULL;
+ * autotune != 0 runs doma_autotune first.  Transient decks are rejected. */
 int doma_run_deck(const char *path, const char *output_base, int32_t precision,
                   const doma_tuning *tuning, int32_t autotune, double *phi, int64_t phi_len,
                   doma_stats *stats, int32_t *converged);

@@ -28,7 +28,8 @@ class OracleProblem(C.Structure):
         ("lx", _dp), ("ly", _dp), ("lz", _dp), ("groups", C.c_int), ("scheme", C.c_int),
         ("num_mats", C.c_int), ("mat", _ip), ("sigt", _dp), ("sigs", _dp), ("src", _dp),
         ("has_src", _u8p), ("dirs", C.c_int), ("mu", _dp), ("eta", _dp), ("xi", _dp),
-        ("w", _dp),
+        ("w", _dp), ("chi", _dp), ("prod", _dp), ("prompt", _dp), ("has_chi", _u8p),
+        ("has_prod", _u8p),
     ]
 
 
@@ -36,7 +37,8 @@ class OracleStats(C.Structure):
     _fields_ = [
         ("success", C.c_int), ("mg_iterations", C.c_int), ("mg_residual", C.c_double),
         ("sweeps", C.c_int), ("si_iterations_last", C.c_int * 64),
-        ("si_residual_last", C.c_double * 64),
+        ("si_residual_last", C.c_double * 64), ("k_eff", C.c_double),
+        ("k_residual", C.c_double),
     ]
 
 
@@ -71,6 +73,9 @@ def lib():
         h.oracle_solve_fixed_source.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
                                                 C.c_double, C.c_int, C.c_int, _dp,
                                                 C.POINTER(OracleStats)]
+        h.oracle_solve_eigenvalue.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
+                                              C.c_double, C.c_int, C.c_double, C.c_int, _dp,
+                                              _dp, C.POINTER(OracleStats)]
         h.oracle_source_iteration.argtypes = [C.POINTER(OracleProblem), C.c_int, _dp, C.c_double,
                                               C.c_int, C.c_int, _dp, _ip, _dp, _dp]
         _LIB = h
@@ -91,6 +96,11 @@ class _Bound:
                                p.scheme, p.num_mats, p.mat.ctypes.data_as(_ip), _d(p.sigt),
                                _d(p.sigs), _d(p.src), p.has_src.ctypes.data_as(_u8p), p.dirs,
                                _d(p.mu), _d(p.eta), _d(p.xi), _d(p.w))
+        f = p.fission
+        if f is not None:
+            self.s.chi, self.s.prod, self.s.prompt = _d(f.chi), _d(f.prod), _d(f.prompt)
+            self.s.has_chi = f.has_chi.ctypes.data_as(_u8p)
+            self.s.has_prod = f.has_prod.ctypes.data_as(_u8p)
 
 
 def legendre(n):
@@ -138,6 +148,25 @@ def solve_fixed_source(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1
              "si_iterations": list(st.si_iterations_last[: b.p.groups]),
              "si_residual": list(st.si_residual_last[: b.p.groups])}
     return phi.reshape(b.p.groups, b.p.N), stats
+
+
+def solve_eigenvalue(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol, nthreads=1):
+    """Power iteration: (k, phi[G, N], stats)."""
+    b = _Bound(problem)
+    if b.p.fission is None:
+        raise ValueError("an eigenvalue problem needs fission data")
+    phi = np.zeros(b.p.groups * b.p.N)
+    st, k = OracleStats(), C.c_double()
+    ok = lib().oracle_solve_eigenvalue(C.byref(b.s), fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol,
+                                       nthreads, _d(phi), C.byref(k), C.byref(st))
+    if ok < 0:
+        raise RuntimeError("oracle_solve_eigenvalue failed")
+    stats = {"success": bool(st.success), "mg_iterations": st.mg_iterations,
+             "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+             "si_iterations": list(st.si_iterations_last[: b.p.groups]),
+             "si_residual": list(st.si_residual_last[: b.p.groups]),
+             "k_eff": st.k_eff, "k_residual": st.k_residual}
+    return k.value, phi.reshape(b.p.groups, b.p.N), stats
 
 
 def source_iteration(problem, g, qmg, fs_tol, fs_max_it, nthreads=1, trace=False):

@@ -522,19 +522,47 @@ typedef struct {
   int sweeps;
 } solver_t;
 
+/* Prompt fission term: 0.25 (2D, 1D: 0.5) * f * chi_g * nuSf_g' * phi_g', / pi in
+ * 2D/3D, with f = 1 - sum_d beta (TransportSolver3D.cpp:377-387,
+ * TransportSolver2D.cpp:377-387, TransportSolver1D.cpp:410-420). */
+static double fission_term(const oracle_problem *p, double f, double chi, double prod,
+                           double phi)
+{
+  const double t = (p->dims == 3 ? 0.25 : 0.5) * f * chi * prod * phi;
+  return p->dims == 1 ? t : t / M_PI;
+}
+
 /* updateMultigroupSource (TransportSolver3D.cpp:339-402, TransportSolver2D.cpp:
- * 339-402, TransportSolver1D.cpp:369-435), fixed-source terms only. */
-static void update_mg_source(solver_t *S, int g)
+ * 339-402, TransportSolver1D.cpp:369-435): external source, in-scatter and prompt
+ * fission.  With eigen != 0, updateMultigroupSourceEigen (TransportSolver3D.cpp:
+ * 404-430, TransportSolver2D.cpp:404-430, TransportSolver1D.cpp:437-466):
+ * in-scatter and fission / k on top of the iteration source left by the last
+ * scattering update in 2D/3D (which the reference does not clear), from zero in 1D. */
+static void update_mg_source_k(solver_t *S, int g, int eigen, double k)
 {
   const oracle_problem *p = S->p;
   const long N = ncells(p);
   const int G = p->groups;
   for (long c = 0; c < N; ++c) {
     const int m = p->mat[c];
-    double acc = p->has_src[m] ? norm_src(p, p->src[(long)m * G + g]) : 0.0;
-    for (int gp = 0; gp < G; ++gp)
+    const int fis = p->chi && p->has_chi[m];
+    double acc;
+    if (eigen)
+      acc = p->dims == 1 ? 0.0 : S->q[c];
+    else
+      acc = p->has_src[m] ? norm_src(p, p->src[(long)m * G + g]) : 0.0;
+    for (int gp = 0; gp < G; ++gp) {
+      const double f = S->phi[(long)gp * N + c];
       if (gp != g)
-        acc += norm_term(p, p->sigs[((long)m * G + g) * G + gp], S->phi[(long)gp * N + c]);
+        acc += norm_term(p, p->sigs[((long)m * G + g) * G + gp], f);
+      if (fis) {
+        const double chi = p->chi[(long)m * G + g], pr = p->prod[(long)m * G + gp];
+        if (eigen)
+          acc += fission_term(p, 1.0, chi, pr, f) / k;
+        else
+          acc += fission_term(p, p->prompt[(long)m * G + gp], chi, pr, f);
+      }
+    }
     S->dphi[c] = 0.0;
     if (p->dims == 1)
       S->qmg[c] = acc;
@@ -544,6 +572,8 @@ static void update_mg_source(solver_t *S, int g)
     S->phi[(long)g * N + c] = 0.0;
   }
 }
+
+static void update_mg_source(solver_t *S, int g) { update_mg_source_k(S, g, 0, 0.0); }
 
 /* computeScatteringResidual: increment form in 2D/3D (TransportSolver3D.cpp:
  * 547-563), standard form in 1D (TransportSolver1D.cpp:822-838). */
@@ -687,6 +717,81 @@ int oracle_solve_fixed_source(const oracle_problem *p, double fs_tol, int fs_max
     stats->mg_iterations = mg_it;
     stats->mg_residual = r;
     stats->sweeps = S.sweeps;
+  }
+  solver_free(&S);
+  return success;
+}
+
+/* updateEigenvalue (TransportSolver3D.cpp:432-456, TransportSolver2D.cpp:432-456,
+ * TransportSolver1D.cpp:468-493): returns the new k. */
+static double update_eigenvalue(solver_t *S, double k)
+{
+  const oracle_problem *p = S->p;
+  const long N = ncells(p);
+  const int G = p->groups;
+  double num = 0.0, den = 0.0;
+  for (long c = 0; c < N; ++c) {
+    const int m = p->mat[c];
+    if (!p->has_prod[m]) continue;
+    const double v = measure(p, c);
+    for (int g = 0; g < G; ++g) {
+      const double pr = p->prod[(long)m * G + g];
+      num = num + v * pr * S->phi[(long)g * N + c];
+      if (p->dims == 1)
+        den = den + v * pr * S->prev[(long)g * N + c];
+      else
+        den = den + v * pr * S->prev[(long)g * N + c] / k;
+    }
+  }
+  return p->dims == 1 ? k * num / den : num / den;
+}
+
+/* solveEigenvalue (TransportSolver3D.cpp:29-86; TransportSolver1D.cpp:36-90):
+ * phi = 1, k = 1, k_prev = 0 (initializeSolve, constructor); per pass the eigen
+ * source and source iteration of every group, then k and the shape residual;
+ * a failed source iteration ends the solve with false. */
+int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_it,
+                            double mg_tol, int mg_max_it, double k_tol, int nthreads,
+                            double *phi_out, double *k_eff, oracle_stats *stats)
+{
+  if (!p->chi || !p->prod || !p->has_chi || !p->has_prod) return -1;
+  solver_t S;
+  if (solver_init(&S, p, nthreads)) { solver_free(&S); return -1; }
+  const long N = ncells(p);
+  const int G = p->groups;
+  for (long e = 0; e < (long)G * N; ++e) S.phi[e] = 1.0;
+  double k = 1.0, k_prev = 0.0, shape = 0.0, kdiff = 0.0;
+  int mg_it = 0, ok = 1;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  do {
+    for (int g = 0; g < G; ++g) {
+      update_mg_source_k(&S, g, 1, k);
+      int its = 0;
+      double res = 0.0;
+      int conv = source_iteration(&S, g, fs_tol, fs_max_it, &its, &res, NULL);
+      if (stats && g < 64) {
+        stats->si_iterations_last[g] = its;
+        stats->si_residual_last[g] = res;
+      }
+      if (!conv) { ok = 0; break; }
+    }
+    if (!ok) break;
+    k_prev = k;
+    k = update_eigenvalue(&S, k);
+    shape = mg_residual(&S);
+    kdiff = fabs(k - k_prev);
+    mg_it++;
+  } while (mg_it < mg_max_it && (mg_tol < shape || k_tol < kdiff));
+  int success = ok && (mg_it < mg_max_it);
+  if (phi_out) memcpy(phi_out, S.phi, sizeof(double) * (size_t)G * N);
+  if (k_eff) *k_eff = k;
+  if (stats) {
+    stats->success = success;
+    stats->mg_iterations = mg_it;
+    stats->mg_residual = shape;
+    stats->sweeps = S.sweeps;
+    stats->k_eff = k;
+    stats->k_residual = kdiff;
   }
   solver_free(&S);
   return success;

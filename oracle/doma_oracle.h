@@ -20,6 +20,10 @@
  *     so it cannot be built here; 1D is pinned by the exact DD recurrence in a
  *     pure absorber, the cell balance identity, mirror symmetry of symmetric
  *     decks and the infinite-medium limit.  The 1D DSA path is not restated.
+ *   - fission source and power iteration: 2D/3D against the reference's own
+ *     compiled solveFixedSource/solveEigenvalue on the fission decks
+ *     (tests/golden/decks); 1D against the dominant eigenvalue of the assembled
+ *     DD transport operator (numpy, tiny meshes) and the infinite-medium k.
  */
 #ifndef DOMA_ORACLE_H
 #define DOMA_ORACLE_H
@@ -51,6 +55,14 @@ typedef struct oracle_problem {
   const double *eta;
   const double *xi;
   const double *w;          /* [num_oct][dirs] */
+  /* Fission data (NULL chi: none).  chi/prod/prompt [num_mats][groups]: prompt
+   * spectrum, nu*sigma_f, prompt fraction 1 - sum_d beta (1 without delayed
+   * groups); has_chi/has_prod [num_mats]: 0 where the reference's vector is empty. */
+  const double *chi;
+  const double *prod;
+  const double *prompt;
+  const uint8_t *has_chi;
+  const uint8_t *has_prod;
 } oracle_problem;
 
 typedef struct oracle_stats {
@@ -60,6 +72,8 @@ typedef struct oracle_stats {
   int sweeps;               /* total sweeps performed */
   int si_iterations_last[64];  /* SI count per group in the last MG pass */
   double si_residual_last[64];
+  double k_eff;             /* oracle_solve_eigenvalue only */
+  double k_residual;
 } oracle_stats;
 
 /* Quadrature (AngularPolynomials.cpp:8-111, GLAngularQuadrature.cpp:3-20,
@@ -102,6 +116,12 @@ int oracle_sweep(const oracle_problem *p, int g, const double *q, double *dphi, 
 int oracle_solve_fixed_source(const oracle_problem *p, double fs_tol, int fs_max_it,
                               double mg_tol, int mg_max_it, int nthreads, double *phi_out,
                               oracle_stats *stats);
+
+/* k-eigenvalue power iteration (TransportSolver3D.cpp:29-86, 404-456 and the
+ * 2D/1D counterparts).  Needs p->chi.  Writes k to *k_eff and stats. */
+int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_it,
+                            double mg_tol, int mg_max_it, double k_tol, int nthreads,
+                            double *phi_out, double *k_eff, oracle_stats *stats);
 
 /* Source iteration of one group in isolation, starting from phi_tot[g] = 0 and
  * the given multigroup source qmg[N] (already scaled by the angular

@@ -9,14 +9,17 @@
 // fields after parsing (ix already multiplied by refinement + 1, scatter
 // matrices dest-major after any transpose):
 //   dims G scheme np na fs_max_it mg_max_it nthreads run
-//   fs_tol mg_tol
+//   fs_tol mg_tol k_tol
 //   nxi dx[nxi] ix[nxi]     (and y, z for dims > 1, > 2)
 //   nblocks blocks[nblocks]
-//   M  then per material: id has_src sigt[G] sigs[G*G] (src[G] if has_src)
-// run = 1 solves the fixed-source problem (2D/3D; the reference's 1D solver needs
-// Eigen) and dumps the outputs at <base> when it converges; run = 0 only builds
-// the mesh and dumps it with zero flux.  Output: <base>.bin = int32 success,
-// f64 phi[G][N]; the solver's stdout goes to <base>.log.
+//   M  then per material: id has_src has_chi has_prod nd sigt[G] sigs[G*G]
+//      (src[G] if has_src) (chi_p[G] if has_chi) (nu_sigma_f[G] if has_prod)
+//      (beta[G*nd])
+// run = 1 solves the fixed-source problem, run = 2 the eigenvalue problem (2D/3D;
+// the reference's 1D solver needs Eigen) and dumps the outputs at <base> when it
+// converges; run = 0 only builds the mesh and dumps it with zero flux.  Output:
+// <base>.bin = int32 success, f64 phi[G][N]; the solver's stdout goes to
+// <base>.log, printed at 17 significant digits (for k).
 #include "BrickMesh1D.h"
 #include "TransportSolver2D.h"
 #include "TransportSolver3D.h"
@@ -97,13 +100,14 @@ main(int argc, char ** argv)
   std::ifstream in(argv[1]);
   const std::string base = argv[2];
   int dims, G, scheme, np, na, fs_max_it, mg_max_it, T, run;
-  double fs_tol, mg_tol;
-  in >> dims >> G >> scheme >> np >> na >> fs_max_it >> mg_max_it >> T >> run >> fs_tol >> mg_tol;
+  double fs_tol, mg_tol, k_tol;
+  in >> dims >> G >> scheme >> np >> na >> fs_max_it >> mg_max_it >> T >> run >> fs_tol >> mg_tol >>
+      k_tol;
   if (!in)
     return 3;
 
   InputParameters params;
-  params._mode = RunMode::FixedSrc;
+  params._mode = run == 2 ? RunMode::Eigen : RunMode::FixedSrc;
   params._num_e_groups = G;
   params._t0 = 0.0;
   params._t1 = 1.0;
@@ -111,7 +115,7 @@ main(int argc, char ** argv)
   params._ic = TransientIC::Zero;
   params._src_it_tol = fs_tol;
   params._gs_tol = mg_tol;
-  params._k_tol = 0.0;
+  params._k_tol = k_tol;
   params._dsa = false;
   params._num_src_it = fs_max_it;
   params._num_mg_it = mg_max_it;
@@ -138,15 +142,20 @@ main(int argc, char ** argv)
   in >> M;
   for (int m = 0; m < M; ++m)
   {
-    unsigned int id;
-    int has;
+    unsigned int id, nd;
+    int has, has_chi, has_prod;
     MaterialProps p;
-    in >> id >> has;
+    in >> id >> has >> has_chi >> has_prod >> nd;
     readn(in, p._g_total, G);
     readn(in, p._g_g_scatter_mat, static_cast<size_t>(G) * G);
     if (has)
       readn(in, p._g_src, G);
-    p._num_d_groups = 0;
+    if (has_chi)
+      readn(in, p._g_chi_p, G);
+    if (has_prod)
+      readn(in, p._g_prod, G);
+    p._num_d_groups = nd;
+    readn(in, p._g_n_beta, static_cast<size_t>(G) * nd);
     params._block_mat_info.emplace(id, p);
   }
   if (!in)
@@ -155,6 +164,7 @@ main(int argc, char ** argv)
   omp_set_num_threads(T);
   std::ostringstream log;
   auto * old = std::cout.rdbuf(log.rdbuf());
+  std::cout.precision(17);
   bool ok = false;
   std::vector<double> phi;
   if (dims == 3)
@@ -168,12 +178,12 @@ main(int argc, char ** argv)
     else if (scheme == 1)
     {
       TransportSolver3D<TWDiamondDifference3D> s(mesh, params, true, T);
-      ok = s.solveFixedSource(base);
+      ok = run == 2 ? s.solveEigenvalue(base) : s.solveFixedSource(base);
     }
     else
     {
       TransportSolver3D<DiamondDifference3D> s(mesh, params, true, T);
-      ok = s.solveFixedSource(base);
+      ok = run == 2 ? s.solveEigenvalue(base) : s.solveFixedSource(base);
     }
     collect(TransportSolver3D<RefDeckProbe>::cells(mesh), G, phi);
   }
@@ -188,12 +198,12 @@ main(int argc, char ** argv)
     else if (scheme == 1)
     {
       TransportSolver2D<TWDiamondDifference2D> s(mesh, params, true, T);
-      ok = s.solveFixedSource(base);
+      ok = run == 2 ? s.solveEigenvalue(base) : s.solveFixedSource(base);
     }
     else
     {
       TransportSolver2D<DiamondDifference2D> s(mesh, params, true, T);
-      ok = s.solveFixedSource(base);
+      ok = run == 2 ? s.solveEigenvalue(base) : s.solveFixedSource(base);
     }
     collect(TransportSolver2D<RefDeckProbe>::cells(mesh), G, phi);
   }

@@ -18,7 +18,7 @@ def _vals(text, kind=float):
 
 
 def read_deck(path):
-    """InputParameters fields of a fixed-source deck, as plain Python values."""
+    """InputParameters fields of a deck, as plain Python values."""
     root = ET.parse(path).getroot()
     s, aq, m = root.find("solver"), root.find("angular_quad"), root.find("mesh")
     G = int(s.get("num_groups"))
@@ -27,6 +27,7 @@ def read_deck(path):
         fs_tol=float(s.get("fs_tol")), mg_tol=float(s.get("mg_tol")),
         fs_max_it=int(s.get("fs_max_it")), mg_max_it=int(s.get("mg_max_it")),
         dsa=s.get("dsa") == "true", np=int(aq.get("np")), na=int(aq.get("na")),
+        k_tol=float(s.get("k_tol")) if s.get("solve_type") == "eigen" else 0.0,
         dims=int(m.get("dims")), refinement=int(m.get("refinement")))
     r = p["refinement"] + 1
     for ax in "xyz"[:p["dims"]]:
@@ -36,7 +37,9 @@ def read_deck(path):
     mats = {}
     for mat in root.find("materials").findall("mat"):
         bid = int(mat.get("block"))
-        d = {"src": None}
+        # delayed groups are read in transient mode only (InputParameters.cpp:291-294)
+        nd = int(mat.get("delayed_groups", "0")) if p["mode"] == "transient" else 0
+        d = {"src": None, "chi": None, "prod": None, "beta": [], "nd": nd}
         for rx in mat.findall("reaction"):
             t = rx.get("type")
             v = _vals(rx.get("mgxs"))
@@ -49,6 +52,36 @@ def read_deck(path):
                 d["sigs"] = a.reshape(-1).tolist()
             elif t == "source":
                 d["src"] = v
+            elif t == "chi_p":
+                d["chi"] = v
+            elif t == "nu_sigma_f":
+                d["prod"] = v
+            elif t == "beta":
+                d["beta"] = v
         mats[bid] = d
     p["materials"] = mats
     return root, p
+
+
+def fission_arrays(p, ids):
+    """chi, prod, prompt [M, G], has_chi, has_prod [M] of the materials `ids`
+    (prompt = 1 - sum_d beta[g][d], TransportSolver3D.cpp:380-386), or None when
+    no material carries fission data."""
+    G = p["G"]
+    mats = [p["materials"][i] for i in ids]
+    if not any(m["chi"] or m["prod"] for m in mats):
+        return None
+    chi = np.array([m["chi"] or [0.0] * G for m in mats])
+    prod = np.array([m["prod"] or [0.0] * G for m in mats])
+    prompt = np.ones((len(ids), G))
+    for r, m in enumerate(mats):
+        if m["nd"] > 0:
+            b = np.array(m["beta"]).reshape(G, m["nd"])
+            for g in range(G):
+                gb = 0.0
+                for d in range(m["nd"]):
+                    gb += b[g, d]
+                prompt[r, g] = 1.0 - gb
+    has_chi = np.array([m["chi"] is not None for m in mats], dtype=np.uint8)
+    has_prod = np.array([m["prod"] is not None for m in mats], dtype=np.uint8)
+    return chi, prod, prompt, has_chi, has_prod

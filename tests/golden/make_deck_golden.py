@@ -21,11 +21,18 @@ stores
 
 For 1D decks only the mesh is built (the reference's 1D solver needs Eigen, which
 is absent): the mesh files are pinned, the flux is left to the oracle.
+
+Eigenvalue decks (solve_type="eigen") run the reference's solveEigenvalue; its
+log is printed at 17 significant digits, from which k and the power-iteration
+count are stored (k_eff, pi_iterations).  Decks whose name ends in "_coarse" are
+the reference deck with fewer cells per region (COARSE below) so that the
+reference's CPU solver finishes in seconds; the deck's physics is unchanged.
 """
 from __future__ import annotations
 
 import hashlib
 import json
+import re
 import subprocess
 import sys
 import tempfile
@@ -54,6 +61,20 @@ DECKS = {
     "1D_reed": "cases/1D_reed.xml",
     "1D_NPRE_555_mp2": "cases/1D_NPRE_555_mp2.xml",
     "1D_NPRE_555_mp3_3g_src": "cases/1D_NPRE_555_mp3_3g_src.xml",
+    "1D_NPRE_555_mp3_3g_eig": "cases/1D_NPRE_555_mp3_3g_eig.xml",
+    "2D_assembly_eig": "cases/2D_assembly_eig.xml",
+    "2D_assembly_eig_coarse": "cases/2D_assembly_eig.xml",
+    "3D_assembly_fs_coarse": "cases/3D_assembly_fs.xml",
+    "3D_assembly_eig_coarse": "cases/3D_assembly_eig.xml",
+}
+# mesh attribute -> new value for the "_coarse" decks
+_ASSEMBLY_XY = "8, " + "2, " * 11 + "8"
+COARSE = {
+    "2D_assembly_eig_coarse": {"refinement": ("mesh", "0")},
+    "3D_assembly_fs_coarse": {"ix": ("vals", _ASSEMBLY_XY), "iy": ("vals", _ASSEMBLY_XY),
+                              "iz": ("vals", "4, 1, 4")},
+    "3D_assembly_eig_coarse": {"ix": ("vals", _ASSEMBLY_XY), "iy": ("vals", _ASSEMBLY_XY),
+                               "iz": ("vals", "8")},
 }
 FULL_LIMIT = 40000  # store the whole flux up to this many values
 THREADS = 8
@@ -61,16 +82,19 @@ THREADS = 8
 
 def write_spec(p, path, run):
     lines = [f"{p['dims']} {p['G']} {p['scheme']} {p['np']} {p['na']} {p['fs_max_it']} "
-             f"{p['mg_max_it']} {THREADS} {run}", f"{p['fs_tol']!r} {p['mg_tol']!r}"]
+             f"{p['mg_max_it']} {THREADS} {run}",
+             f"{p['fs_tol']!r} {p['mg_tol']!r} {p['k_tol']!r}"]
     for ax in "xyz"[:p["dims"]]:
         d, n = p["d" + ax], p["i" + ax]
         lines.append(f"{len(d)} " + " ".join(repr(v) for v in d) + " " + " ".join(map(str, n)))
     lines.append(f"{len(p['blocks'])} " + " ".join(map(str, p["blocks"])))
     lines.append(str(len(p["materials"])))
     for bid, d in sorted(p["materials"].items()):
-        has = d["src"] is not None
-        vals = d["sigt"] + d["sigs"] + (d["src"] if has else [])
-        lines.append(f"{bid} {int(has)} " + " ".join(repr(v) for v in vals))
+        flags = [d["src"] is not None, d["chi"] is not None, d["prod"] is not None]
+        vals = d["sigt"] + d["sigs"] + (d["src"] or []) + (d["chi"] or []) + \
+            (d["prod"] or []) + (d["beta"] if d["nd"] else [])
+        lines.append(f"{bid} " + " ".join(str(int(f)) for f in flags) + f" {d['nd']} " +
+                     " ".join(repr(v) for v in vals))
     Path(path).write_text("\n".join(lines) + "\n")
 
 
@@ -80,13 +104,27 @@ def _files(base, dims, G):
     return {n: Path(f"{base}_{n}.txt") for n in names}
 
 
+def _coarsen(name, src, tmp):
+    """The reference deck with the COARSE mesh attributes replaced."""
+    tree = ET.parse(src)
+    mesh = tree.getroot().find("mesh")
+    for tag, (attr, val) in COARSE[name].items():
+        el = mesh if tag == "refinement" else mesh.find(tag)
+        el.set("refinement" if tag == "refinement" else attr, val)
+    out = Path(tmp) / f"{name}.src.xml"
+    tree.write(out, encoding="unicode")
+    return out
+
+
 def make(name, tmp):
     src = REF / DECKS[name]
+    if name in COARSE:
+        src = _coarsen(name, src, tmp)
     root, p = read_deck(src)
-    assert p["mode"] == "steady", name
+    assert p["mode"] in ("steady", "eigen"), name
     OUT.mkdir(exist_ok=True)
     ET.ElementTree(root).write(OUT / f"{name}.xml", encoding="unicode")
-    run = 1 if p["dims"] > 1 else 0
+    run = 0 if p["dims"] == 1 else (2 if p["mode"] == "eigen" else 1)
     spec, base = Path(tmp) / f"{name}.spec", str(Path(tmp) / name)
     write_spec(p, spec, run)
     subprocess.run([str(HARNESS), str(spec), base], check=True)
@@ -124,6 +162,15 @@ def make(name, tmp):
         from make_golden import parse_log  # noqa: E402
         meta["log"] = parse_log(log)
         arrays["si_iterations"] = np.array([a for a, _, _ in meta["log"]["si"]], dtype=np.int64)
+        meta["log"]["si"] = meta["log"]["si"][-p["G"]:]  # the manifest keeps the last pass
+        if run == 2:
+            k = re.findall(r"Converged value of k = (\S+)", log)
+            pi = re.findall(r"PI (?:converged|failed to converge) after (\d+) iterations", log)
+            arrays["pi_iterations"] = int(pi[-1])
+            if k:
+                arrays["k_eff"] = float(k[-1])
+                meta["k_eff"] = float(k[-1])
+            meta["pi_iterations"] = int(pi[-1])
     np.savez_compressed(OUT / f"{name}.npz", **arrays)
     return meta
 

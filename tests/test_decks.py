@@ -27,7 +27,7 @@ REF_CASES = Path("/root/reference/cases")
 CLI = Path(doma_amd.__file__).resolve().parent / "doma"
 
 sys.path.insert(0, str(GOLDEN))
-from deck_xml import read_deck  # noqa: E402  (xml.etree reader, independent of the product)
+from deck_xml import fission_arrays, read_deck  # noqa: E402  (xml.etree reader, independent of the product)
 
 DECK_NAMES = sorted(p.stem for p in DECKS.glob("*.xml"))
 SOLVED = [n for n in DECK_NAMES if not n.startswith("1D")]
@@ -215,6 +215,12 @@ def test_deck_run_matches_reference(name, tmp_path):
     print(f"deck {name}", flush=True)
     conv, phi, st = doma_amd.run_deck(DECKS / f"{name}.xml", base)
     assert conv == bool(ref["ref_success"])
+    if "k_eff" in ref:
+        # eigenvalue decks: same number of power iterations and the same k
+        # (the reference's log prints k at 17 significant digits)
+        assert st["mg_iterations"] == int(ref["pi_iterations"])
+        assert abs(st["k_eff"] - float(ref["k_eff"])) <= 1e-10 * float(ref["k_eff"]), (
+            st["k_eff"], float(ref["k_eff"]))
     if "phi" in ref:
         assert rel_err(phi, ref["phi"]).max() < 1e-10, name
     else:
@@ -232,8 +238,8 @@ def test_deck_run_matches_reference(name, tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", MESH_ONLY)
 def test_deck_run_1d(name, tmp_path):
-    """1D decks: mesh files against the reference's; the flux against the oracle
-    (the reference's 1D solver needs Eigen and is not built here)."""
+    """1D decks: mesh files against the reference's; the flux (and k) against the
+    oracle (the reference's 1D solver needs Eigen and is not built here)."""
     from oracle import binding as ob
 
     ref = np.load(DECKS / f"{name}.npz")
@@ -260,8 +266,16 @@ def test_deck_run_1d(name, tmp_path):
     mu, eta, xi, w = workloads.quadrature_1d(p["np"])
     prob = capi.Problem(1, lx, np.ones(1), np.ones(1), G, capi.SCHEME_DD, mat, sigt, sigs, src,
                         has, mu, eta, xi, w)
-    ophi, ost = ob.solve_fixed_source(prob.normalized(), p["fs_tol"], p["fs_max_it"],
-                                      p["mg_tol"], p["mg_max_it"])
+    f = fission_arrays(p, ids)
+    if f is not None:
+        prob.fission = capi.Fission(*f)
+    if p["mode"] == "eigen":
+        k, ophi, ost = ob.solve_eigenvalue(prob.normalized(), p["fs_tol"], p["fs_max_it"],
+                                           p["mg_tol"], p["mg_max_it"], p["k_tol"], nthreads=8)
+        assert abs(st["k_eff"] - k) <= 1e-10 * k, (st["k_eff"], k)
+    else:
+        ophi, ost = ob.solve_fixed_source(prob.normalized(), p["fs_tol"], p["fs_max_it"],
+                                          p["mg_tol"], p["mg_max_it"])
     assert conv == ost["success"]
     assert rel_err(phi, ophi).max() < 1e-10
     assert st["si_iterations"] == ost["si_iterations"]
