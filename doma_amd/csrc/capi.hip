@@ -496,6 +496,21 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     if (rc)
       return rc;
     DOMA_HIP(hipEventRecord(c->ev1, c->stream));
+    if (c->dsa && c->dims == 1)
+    {
+      // syntheticAcceleration follows the sweep (TransportSolver1D.cpp:513-514).
+      DsaArgs d{};
+      d.dirs = c->dirs;
+      d.dirtab = c->d_dirtab;
+      d.beta_left = c->dsa_beta[0];
+      d.beta_right = c->dsa_beta[1];
+      d.work = c->d_dsa + c->N;
+      d.out = c->d_dsa;
+      af.eigen = c->solving_eigen;
+      if ((rc = launch_dsa1d(af, d, c->stream)))
+        return rc;
+      af.dsa = c->d_dsa;
+    }
     if ((rc = launch_finish(af, c->red_blocks, c->stream)) ||
         (rc = launch_residual_final(af, c->red_blocks, c->stream)))
       return rc;
@@ -707,10 +722,11 @@ create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
     dirtab[4 * t + 2] = d->dims > 1 ? std::fabs(d->xi[t]) : 0.0;
     dirtab[4 * t + 3] = d->w[t];
   }
-  // 1D keeps the signed cosine (DiamondDifference1D takes |mu| itself).
+  // 1D DSA boundary coefficients: sum of mu w over each half range, in the
+  // reference's order (TransportSolver1D.cpp:763-766, 779-782).
   if (d->dims == 1)
     for (int t = 0; t < noct * d->dirs; ++t)
-      dirtab[4 * t + 0] = std::fabs(d->mu[t]);
+      c->dsa_beta[t < d->dirs ? 0 : 1] += d->mu[t] * d->w[t];
 
   const size_t N = (size_t)c->N;
   if ((rc = dev_alloc(&c->d_lx, d->nx)) || (rc = dev_alloc(&c->d_ly, ny)) ||
@@ -793,6 +809,7 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_prompt);
   dev_free(c->d_has_chi);
   dev_free(c->d_has_prod);
+  dev_free(c->d_dsa);
   dev_free(c->d_sig);
   dev_free(c->d_q);
   dev_free(c->d_qmg);
@@ -912,6 +929,34 @@ doma_set_fission(doma_ctx * c, int32_t M, const double * chi, const double * pro
   DOMA_HIP(hipMemcpy(c->d_has_chi, has_chi, (size_t)M, hipMemcpyHostToDevice));
   DOMA_HIP(hipMemcpy(c->d_has_prod, has_prod, (size_t)M, hipMemcpyHostToDevice));
   c->fission = true;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_dsa(doma_ctx * c, int32_t enable)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!enable)
+  {
+    c->dsa = 0;
+    return DOMA_OK;
+  }
+  if (c->dims != 1)
+    return fail(DOMA_ERR_INVALID, "diffusion synthetic acceleration is implemented for 1D only");
+  if (c->N < 2)
+    return fail(DOMA_ERR_INVALID, "diffusion synthetic acceleration needs at least two cells");
+  // Void check of the deck reader (InputParameters.cpp:403-414).
+  std::vector<double> sigt((size_t)c->M * c->G);
+  DOMA_HIP(hipMemcpy(sigt.data(), c->d_sigt, sizeof(double) * sigt.size(), hipMemcpyDeviceToHost));
+  for (double t : sigt)
+    if (t < 1e-6)
+      return fail(DOMA_ERR_INVALID, "Void region detected! You cannot use synthetic acceleration "
+                                    "when cross sections are close to zero.");
+  if (!c->d_dsa && (rc = dev_alloc(&c->d_dsa, 5 * (size_t)c->N + 6 * ((size_t)c->N + 1))))
+    return rc;
+  c->dsa = 1;
   return DOMA_OK;
 }
 
@@ -1113,6 +1158,12 @@ doma_solve_eigenvalue(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_
     return rc;
   double k = 1.0, k_prev = 0.0, shape = 0.0, kdiff = 0.0;
   int mg_it = 0, ok = 1;
+  struct EigenScope  // the DSA scheme follows the run mode (TransportSolver1D.cpp:603-606)
+  {
+    doma_ctx * c;
+    explicit EigenScope(doma_ctx * x) : c(x) { c->solving_eigen = 1; }
+    ~EigenScope() { c->solving_eigen = 0; }
+  } scope(c);
   do
   {
     for (int g = 0; g < c->G; ++g)

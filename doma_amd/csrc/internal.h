@@ -152,6 +152,18 @@ struct IterArgs
   const uint8_t * has_prod;// [M]
   int eigen;               // 1: power-iteration source (fission / k, no external source)
   double k;                // current eigenvalue (eigen != 0)
+  const double * dsa;      // [N] 1D: accelerated flux replacing the swept one (or null)
+};
+
+// 1D diffusion synthetic acceleration of one sweep (see iterate.hip).
+struct DsaArgs
+{
+  int dirs;                // directions per half range
+  const double * dirtab;   // [2*dirs][4]: |mu|, -, -, w
+  double beta_left;        // sum_{mu > 0} mu w
+  double beta_right;       // sum_{mu < 0} mu w
+  double * work;           // [4 N + 6 (N + 1)] moments, system, solution
+  double * out;            // [N] accelerated scalar flux
 };
 
 int launch_mg_source(const IterArgs & a, hipStream_t s);
@@ -163,6 +175,9 @@ int launch_mg_residual(const IterArgs & a, int nblocks, hipStream_t s);
 // Fission production sums of updateEigenvalue: res[4] = sum V nuSf phi,
 // res[5] = sum V nuSf prev (/ k in 2D/3D).  Uses the reduction partials.
 int launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s);
+// 1D DSA after a sweep: moments from the direction partials, the tridiagonal
+// diffusion system, its solve, the cell-average flux into d.out.
+int launch_dsa1d(const IterArgs & a, const DsaArgs & d, hipStream_t s);
 int launch_fill(double * p, size_t n, double v, hipStream_t s);
 // Batched finish/residual over problem slots (see iterate.hip), training-data
 // snapshots and targets.
@@ -210,6 +225,10 @@ struct doma_ctx
   double * d_src = nullptr;
   uint8_t * d_has_src = nullptr;
   bool fission = false;  // doma_set_fission data present for the current materials
+  int dsa = 0;            // 1D diffusion synthetic acceleration (doma_set_dsa)
+  int solving_eigen = 0;  // inside doma_solve_eigenvalue (selects the DSA scheme)
+  double dsa_beta[2] = {0.0, 0.0};
+  double * d_dsa = nullptr;  // DSA work [4 N + 6 (N + 1)] + output [N]
   double * d_chi = nullptr;
   double * d_prod = nullptr;
   double * d_prompt = nullptr;

@@ -153,8 +153,11 @@ finish_body(const IterArgs & a, int block, int nblocks)
   for (int c = block * RED_THREADS + threadIdx.x; c < a.N; c += stride)
   {
     double d = 0.0;
-    for (int s = 0; s < a.ndirsets; ++s)
-      d += a.part[(size_t)s * a.N + c];
+    if (a.dsa)
+      d = a.dsa[c];
+    else
+      for (int s = 0; s < a.ndirsets; ++s)
+        d += a.part[(size_t)s * a.N + c];
     // After a learned correction e the sweep's source was S (d_last + e); the
     // increment from the corrected flux is K (d_last + e) - e.
     if (a.corr)
@@ -407,6 +410,177 @@ rate_final_kernel(IterArgs a, int nblocks)
   }
 }
 
+// syntheticAcceleration (TransportSolver1D.cpp:587-800; Alcouffe, NSE 64 (1977)
+// 344, with Larsen's boundary conditions, NSE 82 (1982) 47): one work-group.
+// Moments of the sweep from the direction partials part[t][c] = w_t psi_t(c)
+// (t < dirs: mu > 0, swept left to right): phi = sum_t part, J = sum_t mu_t part,
+// edge scalar fluxes from the cell-centre fluxes tallied onto downwind edges
+// (DiamondDifference1D.h:41-43, CartesianCell1D.cpp:116-120).  Rows m = 1 .. N-1
+// couple cells m-1 and m: fixed-source mode uses the source correction (Eq. 22,
+// 23), eigenvalue mode the modified diffusion coefficients D = -h J / (sfR - sfL)
+// unless one is <= 0, then the removal correction.  The (N+1)-unknown system is
+// solved by Gaussian elimination with partial pivoting (the tridiagonal form of
+// LAPACK's dgtsv) in one lane; the cell flux is the average of its two edges.
+__global__ __launch_bounds__(RED_THREADS) void
+dsa1d_kernel(IterArgs a, DsaArgs d)
+{
+#pragma clang fp contract(off)
+  const int N = a.N, n = N + 1, G = a.G, g = a.g, D = d.dirs;
+  double * phi = d.work;
+  double * cur = phi + N;
+  double * sfr = cur + N;
+  double * sfl = sfr + N;
+  double * lo = sfl + N;   // A[m][m-1]
+  double * di = lo + n;    // A[m][m]
+  double * up = di + n;    // A[m][m+1]
+  double * up2 = up + n;   // second super-diagonal created by row interchanges
+  double * rhs = up2 + n;
+  double * x = rhs + n;
+  for (int c = threadIdx.x; c < N; c += RED_THREADS)
+  {
+    // Edge e collects w psi of cell e-1 for mu > 0 and of cell e for mu < 0
+    // (CartesianCell1D.cpp:116-120); the cell's right edge is c+1, its left c.
+    double f = 0.0, j = 0.0, r = 0.0, l = 0.0;
+    for (int t = 0; t < 2 * D; ++t)
+    {
+      const double v = a.part[(size_t)t * N + c];
+      const double mu = t < D ? d.dirtab[4 * t] : -d.dirtab[4 * t];
+      f += v;
+      j += mu * v;
+      if (t < D)
+      {
+        r += v;
+        if (c > 0)
+          l += a.part[(size_t)t * N + c - 1];
+      }
+      else
+      {
+        l += v;
+        if (c + 1 < N)
+          r += a.part[(size_t)t * N + c + 1];
+      }
+    }
+    phi[c] = f;
+    cur[c] = j;
+    sfr[c] = r;
+    sfl[c] = l;
+  }
+  __syncthreads();
+  auto tot = [&](int c) { return a.sigt[(size_t)a.mat[c] * G + g]; };
+  auto self = [&](int c) { return a.sigs[((size_t)a.mat[c] * G + g) * G + g]; };
+  // sigma_r at the edge between cells i and i + 1 (Eq. 23a).
+  auto sig_r = [&](int i) {
+    double sr = (tot(i) - self(i)) * a.lx[i] * phi[i];
+    sr += (tot(i + 1) - self(i + 1)) * a.lx[i + 1] * phi[i + 1];
+    sr *= 0.5 / sfr[i];
+    return sr;
+  };
+  bool removal = false;
+  if (a.eigen)
+  {
+    int bad = 0;
+    for (int i = threadIdx.x; i < N - 1; i += RED_THREADS)
+    {
+      double Di = -1.0 * a.lx[i] * cur[i];
+      Di /= (sfr[i] - sfl[i]);
+      double Di1 = -1.0 * a.lx[i + 1] * cur[i + 1];
+      Di1 /= (sfr[i + 1] - sfl[i + 1]);
+      if (Di <= 0.0 || Di1 <= 0.0)
+        bad = 1;
+      else
+      {
+        const int m = i + 1;
+        lo[m] = -1.0 * Di / a.lx[i];
+        di[m] = (Di / a.lx[i]) + (Di1 / a.lx[i + 1]) + sig_r(i);
+        up[m] = -1.0 * Di1 / a.lx[i + 1];
+        rhs[m] = a.qmg[i] * a.lx[i] + a.qmg[i + 1] * a.lx[i + 1];
+      }
+    }
+    removal = __syncthreads_or(bad) != 0;
+  }
+  if (!a.eigen || removal)
+    for (int i = threadIdx.x; i < N - 1; i += RED_THREADS)
+    {
+      const int m = i + 1;
+      const double Dh = 1.0 / 3.0 / tot(i) / a.lx[i];
+      const double Dh1 = 1.0 / 3.0 / tot(i + 1) / a.lx[i + 1];
+      double corr = cur[i + 1] - cur[i];
+      corr += Dh1 * (sfr[i + 1] - sfl[i + 1]);
+      corr -= Dh * (sfr[i] - sfl[i]);
+      const double qqh = a.qmg[i] * a.lx[i] + a.qmg[i + 1] * a.lx[i + 1];
+      lo[m] = -1.0 * Dh;
+      up[m] = -1.0 * Dh1;
+      if (a.eigen)  // removal correction
+      {
+        corr /= sfr[i];
+        di[m] = Dh + Dh1 + sig_r(i) + corr;
+        rhs[m] = qqh;
+      }
+      else  // source correction
+      {
+        di[m] = Dh + Dh1 + sig_r(i);
+        rhs[m] = qqh - corr;
+      }
+    }
+  if (threadIdx.x == 0)
+  {
+    // Boundary rows (Larsen Eq. 34, 37).
+    const double hd0 = 0.5 / 3.0 / tot(0) / a.lx[0];
+    const double es0 = 0.125 * (tot(0) - self(0)) * a.lx[0];
+    lo[0] = 0.0;
+    di[0] = d.beta_left + hd0 + es0;
+    up[0] = -1.0 * hd0 + es0;
+    rhs[0] = -1.0 * a.qmg[0];
+    const int I = N - 1;
+    const double hdI = 0.5 / 3.0 / tot(I) / a.lx[I];
+    const double esI = 0.125 * (tot(I) - self(I)) * a.lx[I];
+    di[N] = d.beta_right - hdI - esI;
+    lo[N] = hdI - esI;
+    up[N] = 0.0;
+    rhs[N] = a.qmg[I];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    // Forward elimination with partial pivoting between rows i and i + 1.
+    for (int i = 0; i < n - 1; ++i)
+    {
+      up2[i] = 0.0;
+      if (fabs(di[i]) >= fabs(lo[i + 1]))
+      {
+        const double f = lo[i + 1] / di[i];
+        di[i + 1] = di[i + 1] - f * up[i];
+        rhs[i + 1] = rhs[i + 1] - f * rhs[i];
+      }
+      else
+      {
+        // Rows i and i + 1 swap: row i becomes [lo[i+1], di[i+1], up[i+1]].
+        const double f = di[i] / lo[i + 1];
+        di[i] = lo[i + 1];
+        const double t = di[i + 1];
+        di[i + 1] = up[i] - f * t;
+        if (i < n - 2)
+        {
+          up2[i] = up[i + 1];
+          up[i + 1] = -f * up2[i];
+        }
+        up[i] = t;
+        const double r = rhs[i];
+        rhs[i] = rhs[i + 1];
+        rhs[i + 1] = r - f * rhs[i + 1];
+      }
+    }
+    x[n - 1] = rhs[n - 1] / di[n - 1];
+    if (n > 1)
+      x[n - 2] = (rhs[n - 2] - up[n - 2] * x[n - 1]) / di[n - 2];
+    for (int i = n - 3; i >= 0; --i)
+      x[i] = (rhs[i] - up[i] * x[i + 1] - up2[i] * x[i + 2]) / di[i];
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < N; c += RED_THREADS)
+    d.out[c] = 0.5 * (x[c] + x[c + 1]);
+}
+
 __global__ void
 fill_kernel(double * p, size_t n, double v)
 {
@@ -530,6 +704,13 @@ launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s)
 {
   hipLaunchKernelGGL(fission_rate_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
   hipLaunchKernelGGL(rate_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_dsa1d(const IterArgs & a, const DsaArgs & d, hipStream_t s)
+{
+  hipLaunchKernelGGL(dsa1d_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, d);
   return DOMA_OK;
 }
 

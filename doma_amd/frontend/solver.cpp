@@ -151,6 +151,8 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
   if (rc == DOMA_OK && fission)
     rc = doma_set_fission(ctx, M, chi.data(), prod.data(), prompt.data(), has_chi.data(),
                           has_prod.data());
+  if (rc == DOMA_OK && _use_dsa)
+    rc = doma_set_dsa(ctx, 1);
   if (rc == DOMA_OK && (_opt.tuning.brick || _opt.tuning.threads || _opt.tuning.exact_arith ||
                         _opt.tuning.schedule || _opt.tuning.octants || _opt.tuning.dirs_per_lane ||
                         _opt.tuning.shape))
@@ -326,6 +328,7 @@ TransportSolver1D<T>::TransportSolver1D(BrickMesh1D & mesh, const InputParameter
                                         bool verbose, bool dsa, unsigned int)
   : SolverCore(mesh, params, verbose, T::scheme), _angular_quad(2u * params._num_polar), _dsa(dsa)
 {
+  _use_dsa = dsa;
 }
 
 template <typename T>
@@ -334,9 +337,6 @@ TransportSolver1D<T>::solveFixedSource(const std::string & output_file_base, con
 {
   if (!fixedSourceOnly())
     return false;
-  if (_dsa)
-    return refuse("Diffusion synthetic acceleration is outside this build's scope; solve with "
-                  "dsa=\"false\".");
   const int n = (int)_angular_quad.order();
   std::vector<double> mu(n), eta(n, 0.0), xi(n, 0.0), w(n);
   for (int i = 0; i < n; ++i)
@@ -352,9 +352,6 @@ bool
 TransportSolver1D<T>::solveEigenvalue(const std::string & output_file_base)
 {
   _error.clear();
-  if (_dsa)
-    return refuse("Diffusion synthetic acceleration is outside this build's scope; solve with "
-                  "dsa=\"false\".");
   const int n = (int)_angular_quad.order();
   std::vector<double> mu(n), eta(n, 0.0), xi(n, 0.0), w(n);
   for (int i = 0; i < n; ++i)

@@ -51,6 +51,7 @@ protected:
   const InputParameters & _params;
   bool _verbose;
   int _scheme;
+  bool _use_dsa = false;  // 1D diffusion synthetic acceleration
   SolverOptions _opt;
   doma_stats _stats{};
   std::string _error;

@@ -125,6 +125,14 @@ int doma_set_materials(doma_ctx *ctx, const int32_t *mat, int32_t num_mats, cons
 int doma_set_fission(doma_ctx *ctx, int32_t num_mats, const double *chi_p, const double *prod,
                      const double *prompt, const uint8_t *has_chi, const uint8_t *has_prod);
 
+/* 1D diffusion synthetic acceleration (TransportSolver1D.cpp:587-800, the
+ * reference's dsa="true"): after every sweep the flux is replaced by the solution
+ * of Alcouffe's diffusion system (source correction in fixed-source solves,
+ * modified diffusion coefficients or removal correction in eigenvalue solves),
+ * solved on the device.  1D contexts only; rejects void cross sections (< 1e-6,
+ * InputParameters.cpp:403-414).  enable = 0 turns it off. */
+int doma_set_dsa(doma_ctx *ctx, int32_t enable);
+
 int doma_set_tuning(doma_ctx *ctx, const doma_tuning *t);
 int doma_get_tuning(const doma_ctx *ctx, doma_tuning *t);
 

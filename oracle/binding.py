@@ -29,7 +29,7 @@ class OracleProblem(C.Structure):
         ("num_mats", C.c_int), ("mat", _ip), ("sigt", _dp), ("sigs", _dp), ("src", _dp),
         ("has_src", _u8p), ("dirs", C.c_int), ("mu", _dp), ("eta", _dp), ("xi", _dp),
         ("w", _dp), ("chi", _dp), ("prod", _dp), ("prompt", _dp), ("has_chi", _u8p),
-        ("has_prod", _u8p),
+        ("has_prod", _u8p), ("dsa", C.c_int),
     ]
 
 
@@ -76,6 +76,8 @@ def lib():
         h.oracle_solve_eigenvalue.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
                                               C.c_double, C.c_int, C.c_double, C.c_int, _dp,
                                               _dp, C.POINTER(OracleStats)]
+        h.oracle_tridiag_solve.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        h.oracle_dsa_system.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int] + [_dp] * 9
         h.oracle_source_iteration.argtypes = [C.POINTER(OracleProblem), C.c_int, _dp, C.c_double,
                                               C.c_int, C.c_int, _dp, _ip, _dp, _dp]
         _LIB = h
@@ -89,7 +91,7 @@ def _d(a):
 class _Bound:
     """Keeps the numpy arrays alive while the C struct points at them."""
 
-    def __init__(self, p):
+    def __init__(self, p, dsa=False):
         p = p.normalized()
         self.p = p
         self.s = OracleProblem(p.dims, p.nx, p.ny, p.nz, _d(p.lx), _d(p.ly), _d(p.lz), p.groups,
@@ -101,6 +103,7 @@ class _Bound:
             self.s.chi, self.s.prod, self.s.prompt = _d(f.chi), _d(f.prod), _d(f.prompt)
             self.s.has_chi = f.has_chi.ctypes.data_as(_u8p)
             self.s.has_prod = f.has_prod.ctypes.data_as(_u8p)
+        self.s.dsa = int(dsa)
 
 
 def legendre(n):
@@ -135,8 +138,8 @@ def sweep(problem, g, q, nthreads=1):
     return out
 
 
-def solve_fixed_source(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1):
-    b = _Bound(problem)
+def solve_fixed_source(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1, dsa=False):
+    b = _Bound(problem, dsa)
     phi = np.zeros(b.p.groups * b.p.N)
     st = OracleStats()
     ok = lib().oracle_solve_fixed_source(C.byref(b.s), fs_tol, fs_max_it, mg_tol, mg_max_it,
@@ -150,9 +153,10 @@ def solve_fixed_source(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, nthreads=1
     return phi.reshape(b.p.groups, b.p.N), stats
 
 
-def solve_eigenvalue(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol, nthreads=1):
+def solve_eigenvalue(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol, nthreads=1,
+                     dsa=False):
     """Power iteration: (k, phi[G, N], stats)."""
-    b = _Bound(problem)
+    b = _Bound(problem, dsa)
     if b.p.fission is None:
         raise ValueError("an eigenvalue problem needs fission data")
     phi = np.zeros(b.p.groups * b.p.N)
@@ -182,6 +186,25 @@ def source_iteration(problem, g, qmg, fs_tol, fs_max_it, nthreads=1, trace=False
         raise RuntimeError("oracle_source_iteration failed")
     out = (phi, its.value, res.value, bool(conv))
     return out + ((tr[: its.value],) if trace else ())
+
+
+def tridiag_solve(lo, di, up, rhs):
+    """x with A x = rhs, A tridiagonal (lo[i] = A[i][i-1], up[i] = A[i][i+1])."""
+    a = [np.array(v, dtype=np.float64) for v in (lo, di, up, rhs)]
+    x = np.zeros(len(a[1]))
+    lib().oracle_tridiag_solve(len(x), *[_d(v) for v in a], _d(x))
+    return x
+
+
+def dsa_system(problem, g, eigen, phi, cur, sfr, sfl, qmg):
+    """The 1D DSA tridiagonal rows (lo, di, up, rhs) for one sweep's moments."""
+    b = _Bound(problem)
+    n = b.p.N + 1
+    ins = [np.ascontiguousarray(v, dtype=np.float64) for v in (phi, cur, sfr, sfl, qmg)]
+    outs = [np.zeros(n) for _ in range(4)]
+    lib().oracle_dsa_system(C.byref(b.s), g, int(eigen), *[_d(v) for v in ins],
+                            *[_d(v) for v in outs])
+    return tuple(outs)
 
 
 def twdd3d_cell(q, sig, lx, ly, lz, amu, aeta, axi, inflow):

@@ -305,6 +305,7 @@ typedef struct {
   double *swept;  /* [T][N] per-thread swept scalar flux */
   double *fy;     /* [T][nx] */
   double *fz;     /* [T][nx*ny] */
+  double *mom;    /* 1D: [T][N] per-thread current, then [T][N+1] edge scalar fluxes */
 } scratch_t;
 
 static long ncells(const oracle_problem *p) { return (long)p->nx * p->ny * p->nz; }
@@ -374,9 +375,12 @@ static void sweep2d_dir(const oracle_problem *p, int g, const double *q, int slo
   }
 }
 
-/* sweepR / sweepL (TransportSolver1D.cpp:890-915). */
+/* sweepR / sweepL (TransportSolver1D.cpp:890-915).  With cur (DSA), the tallies
+ * of DiamondDifference1D.h:41-43: w mu psi into the cell current and w psi (the
+ * cell-centre flux) onto the downwind edge, edge i+1 for mu > 0 and edge i for
+ * mu < 0 (CartesianCell1D.cpp:116-120). */
 static void sweep1d_dir(const oracle_problem *p, int g, const double *q, int half, double mu,
-                        double wt, double *swept)
+                        double wt, double *swept, double *cur, double *edge)
 {
   const int nx = p->nx, G = p->groups;
   double f = 0.0;
@@ -386,6 +390,10 @@ static void sweep1d_dir(const oracle_problem *p, int g, const double *q, int hal
     double out;
     const double psi = oracle_dd1d_cell(q[i], sig, p->lx[i], mu, f, &out);
     swept[i] += wt * psi;
+    if (cur) {
+      edge[half == 0 ? i + 1 : i] += wt * psi;
+      cur[i] += wt * mu * psi;
+    }
     f = out;
   }
 }
@@ -393,7 +401,7 @@ static void sweep1d_dir(const oracle_problem *p, int g, const double *q, int hal
 static int team(int nthreads) { return nthreads < 1 ? 1 : nthreads; }
 
 static int sweep_with(const oracle_problem *p, int g, const double *q, double *dphi, int T,
-                      scratch_t *s)
+                      scratch_t *s, double *mom_out)
 {
   const long N = ncells(p);
   const int D = p->dirs;
@@ -434,7 +442,9 @@ static int sweep_with(const oracle_problem *p, int g, const double *q, double *d
         tid = omp_get_thread_num();
 #endif
         const int at = half * D + n;
-        sweep1d_dir(p, g, q, half, p->mu[at], p->w[at], s->swept + (size_t)tid * N);
+        double *cur = mom_out ? s->mom + (size_t)tid * N : NULL;
+        double *edge = mom_out ? s->mom + (size_t)T * N + (size_t)tid * (N + 1) : NULL;
+        sweep1d_dir(p, g, q, half, p->mu[at], p->w[at], s->swept + (size_t)tid * N, cur, edge);
       }
     }
   } else {
@@ -450,6 +460,23 @@ static int sweep_with(const oracle_problem *p, int g, const double *q, double *d
     }
     dphi[c] = acc;
   }
+  if (mom_out) {
+    /* moments (TransportSolver1D.cpp:862-882): [0] cell current, [1] scalar flux
+     * of the cell's right edge, [2] of its left edge, each summed over threads */
+    const double *edge = s->mom + (size_t)T * N;
+    for (long c = 0; c < N; ++c) {
+      double j = 0.0, r = 0.0, l = 0.0;
+      for (int t = 0; t < T; ++t) {
+        j += s->mom[(size_t)t * N + c];
+        r += edge[(size_t)t * (N + 1) + c + 1];
+        l += edge[(size_t)t * (N + 1) + c];
+      }
+      mom_out[c] = j;
+      mom_out[N + c] = r;
+      mom_out[2 * N + c] = l;
+    }
+    memset(s->mom, 0, sizeof(double) * (size_t)T * (2 * N + 1));
+  }
   return 0;
 }
 
@@ -459,7 +486,8 @@ static int scratch_alloc(const oracle_problem *p, int T, scratch_t *s)
   s->swept = calloc((size_t)T * N, sizeof(double));
   s->fy = calloc((size_t)T * p->nx, sizeof(double));
   s->fz = calloc((size_t)T * p->nx * p->ny, sizeof(double));
-  return (s->swept && s->fy && s->fz) ? 0 : -1;
+  s->mom = p->dims == 1 ? calloc((size_t)T * (2 * N + 1), sizeof(double)) : NULL;
+  return (s->swept && s->fy && s->fz && (p->dims != 1 || s->mom)) ? 0 : -1;
 }
 
 static void scratch_free(scratch_t *s)
@@ -467,6 +495,7 @@ static void scratch_free(scratch_t *s)
   free(s->swept);
   free(s->fy);
   free(s->fz);
+  free(s->mom);
 }
 
 int oracle_sweep(const oracle_problem *p, int g, const double *q, double *dphi, int nthreads)
@@ -474,7 +503,7 @@ int oracle_sweep(const oracle_problem *p, int g, const double *q, double *dphi, 
   const int T = team(nthreads);
   scratch_t s;
   if (scratch_alloc(p, T, &s)) { scratch_free(&s); return -1; }
-  int rc = sweep_with(p, g, q, dphi, T, &s);
+  int rc = sweep_with(p, g, q, dphi, T, &s, NULL);
   scratch_free(&s);
   return rc;
 }
@@ -519,6 +548,8 @@ typedef struct {
   double *qmg;   /* 1D multigroup source (_mg_source) */
   double *phi;   /* [G][N] total scalar flux */
   double *prev;  /* [G][N] previous MG iterate */
+  double *mom;   /* 1D DSA: [3][N] current, right and left interface fluxes */
+  int eigen;     /* solving the eigenvalue problem (selects the DSA scheme) */
   int sweeps;
 } solver_t;
 
@@ -618,6 +649,130 @@ static void update_scattering_source(solver_t *S, int g)
   }
 }
 
+void oracle_tridiag_solve(int n, double *lo, double *di, double *up, double *rhs, double *x)
+{
+  double *up2 = calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  for (int i = 0; i < n - 1; ++i) {
+    if (fabs(di[i]) >= fabs(lo[i + 1])) {
+      const double f = lo[i + 1] / di[i];
+      di[i + 1] -= f * up[i];
+      rhs[i + 1] -= f * rhs[i];
+    } else {
+      /* interchange rows i and i+1, then eliminate below */
+      const double f = di[i] / lo[i + 1];
+      const double d1 = di[i + 1], u1 = i < n - 2 ? up[i + 1] : 0.0, r0 = rhs[i];
+      di[i] = lo[i + 1];
+      di[i + 1] = up[i] - f * d1;
+      up[i] = d1;
+      if (i < n - 2) {
+        up2[i] = u1;
+        up[i + 1] = -f * u1;
+      }
+      rhs[i] = rhs[i + 1];
+      rhs[i + 1] = r0 - f * rhs[i];
+    }
+  }
+  x[n - 1] = rhs[n - 1] / di[n - 1];
+  if (n > 1) x[n - 2] = (rhs[n - 2] - up[n - 2] * x[n - 1]) / di[n - 2];
+  for (int i = n - 3; i >= 0; --i)
+    x[i] = (rhs[i] - up[i] * x[i + 1] - up2[i] * x[i + 2]) / di[i];
+  free(up2);
+}
+
+/* The diffusion system of syntheticAcceleration (TransportSolver1D.cpp:587-800):
+ * Alcouffe's equations on the N+1 cell edges from a sweep's flux phi, cell
+ * currents cur and edge fluxes sfr/sfl (right/left edge of each cell); fixed-
+ * source mode: source correction (Eq. 22-23); eigenvalue mode: modified
+ * diffusion coefficients D = -h J / (sfR - sfL), or, if any is <= 0, the removal
+ * correction.  Boundary rows after Larsen (Eq. 34, 37). */
+void oracle_dsa_system(const oracle_problem *p, int g, int eigen, const double *phi,
+                       const double *cur, const double *sfr, const double *sfl,
+                       const double *qmg, double *lo, double *di, double *up, double *rhs)
+{
+  const int N = p->nx, G = p->groups, D = p->dirs;
+  const double *l = p->lx;
+  memset(lo, 0, sizeof(double) * (size_t)(N + 1));
+  memset(up, 0, sizeof(double) * (size_t)(N + 1));
+#define TOT(c) (p->sigt[(long)p->mat[c] * G + g])
+#define SELF(c) (p->sigs[((long)p->mat[c] * G + g) * G + g])
+  int removal = 0;
+  if (eigen) {
+    for (int i = 0; i < N - 1; ++i) {
+      double Di = -1.0 * l[i] * cur[i];
+      Di /= (sfr[i] - sfl[i]);
+      double Di1 = -1.0 * l[i + 1] * cur[i + 1];
+      Di1 /= (sfr[i + 1] - sfl[i + 1]);
+      if (Di <= 0.0 || Di1 <= 0.0) { removal = 1; break; }
+      double sr = (TOT(i) - SELF(i)) * l[i] * phi[i];
+      sr += (TOT(i + 1) - SELF(i + 1)) * l[i + 1] * phi[i + 1];
+      sr *= 0.5 / sfr[i];
+      lo[i + 1] = -1.0 * Di / l[i];
+      di[i + 1] = (Di / l[i]) + (Di1 / l[i + 1]) + sr;
+      up[i + 1] = -1.0 * Di1 / l[i + 1];
+      rhs[i + 1] = (qmg[i] * l[i] + qmg[i + 1] * l[i + 1]);
+    }
+  }
+  if (!eigen || removal) {
+    for (int i = 0; i < N - 1; ++i) {
+      const double Dh = 1.0 / 3.0 / TOT(i) / l[i];
+      const double Dh1 = 1.0 / 3.0 / TOT(i + 1) / l[i + 1];
+      double sr = (TOT(i) - SELF(i)) * l[i] * phi[i];
+      sr += (TOT(i + 1) - SELF(i + 1)) * l[i + 1] * phi[i + 1];
+      sr *= 0.5 / sfr[i];
+      double r = cur[i + 1] - cur[i];
+      r += Dh1 * (sfr[i + 1] - sfl[i + 1]);
+      r -= Dh * (sfr[i] - sfl[i]);
+      const double qqh = (qmg[i] * l[i] + qmg[i + 1] * l[i + 1]);
+      lo[i + 1] = -1.0 * Dh;
+      up[i + 1] = -1.0 * Dh1;
+      if (eigen) {
+        r /= sfr[i];
+        di[i + 1] = Dh + Dh1 + sr + r;
+        rhs[i + 1] = qqh;
+      } else {
+        di[i + 1] = Dh + Dh1 + sr;
+        rhs[i + 1] = qqh - r;
+      }
+    }
+  }
+  double bl = 0.0, br = 0.0;
+  for (int t = 0; t < D; ++t) bl += p->mu[t] * p->w[t];
+  for (int t = D; t < 2 * D; ++t) br += p->mu[t] * p->w[t];
+  {
+    rhs[0] = -1.0 * qmg[0];
+    const double hd = 0.5 / 3.0 / TOT(0) / l[0];
+    const double es = 0.125 * (TOT(0) - SELF(0)) * l[0];
+    di[0] = bl + hd + es;
+    up[0] = -1.0 * hd + es;
+  }
+  {
+    const int I = N - 1;
+    rhs[N] = qmg[I];
+    const double hd = 0.5 / 3.0 / TOT(I) / l[I];
+    const double es = 0.125 * (TOT(I) - SELF(I)) * l[I];
+    di[N] = br - hd - es;
+    lo[N] = hd - es;
+  }
+#undef TOT
+#undef SELF
+}
+
+/* syntheticAcceleration: solve the system; the cell flux becomes the average of
+ * its two edge values (TransportSolver1D.cpp:797-799). */
+static void synthetic_acceleration(solver_t *S, int g)
+{
+  const oracle_problem *p = S->p;
+  const int N = p->nx, n = N + 1;
+  double *lo = calloc((size_t)n, sizeof(double)), *di = calloc((size_t)n, sizeof(double));
+  double *up = calloc((size_t)n, sizeof(double)), *rhs = calloc((size_t)n, sizeof(double));
+  double *x = calloc((size_t)n, sizeof(double));
+  oracle_dsa_system(p, g, S->eigen, S->dphi, S->mom, S->mom + N, S->mom + 2 * N, S->qmg, lo, di,
+                    up, rhs);
+  oracle_tridiag_solve(n, lo, di, up, rhs, x);
+  for (int c = 0; c < N; ++c) S->dphi[c] = 0.5 * (x[c] + x[c + 1]);
+  free(lo); free(di); free(up); free(rhs); free(x);
+}
+
 /* sourceIteration (TransportSolver3D.cpp:458-497; TransportSolver1D.cpp:495-541,
  * which first applies updateScatteringSource once). */
 static int source_iteration(solver_t *S, int g, double tol, int max_it, int *its, double *res,
@@ -628,7 +783,9 @@ static int source_iteration(solver_t *S, int g, double tol, int max_it, int *its
   int it = 0;
   double r = 0.0;
   do {
-    sweep_with(p, g, S->q, S->dphi, S->T, &S->s);
+    const int dsa = p->dims == 1 && p->dsa;
+    sweep_with(p, g, S->q, S->dphi, S->T, &S->s, dsa ? S->mom : NULL);
+    if (dsa) synthetic_acceleration(S, g);
     S->sweeps++;
     r = scattering_residual(S, g);
     if (trace) trace[it] = r;
@@ -670,13 +827,14 @@ static int solver_init(solver_t *S, const oracle_problem *p, int nthreads)
   S->qmg = calloc(N, sizeof(double));
   S->phi = calloc((size_t)p->groups * N, sizeof(double));
   S->prev = calloc((size_t)p->groups * N, sizeof(double));
-  if (!S->q || !S->dphi || !S->qmg || !S->phi || !S->prev) return -1;
+  S->mom = calloc((size_t)3 * N, sizeof(double));
+  if (!S->q || !S->dphi || !S->qmg || !S->phi || !S->prev || !S->mom) return -1;
   return scratch_alloc(p, S->T, &S->s);
 }
 
 static void solver_free(solver_t *S)
 {
-  free(S->q); free(S->dphi); free(S->qmg); free(S->phi); free(S->prev);
+  free(S->q); free(S->dphi); free(S->qmg); free(S->phi); free(S->prev); free(S->mom);
   scratch_free(&S->s);
 }
 
@@ -760,6 +918,7 @@ int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_i
   const long N = ncells(p);
   const int G = p->groups;
   for (long e = 0; e < (long)G * N; ++e) S.phi[e] = 1.0;
+  S.eigen = 1;
   double k = 1.0, k_prev = 0.0, shape = 0.0, kdiff = 0.0;
   int mg_it = 0, ok = 1;
   if (stats) memset(stats, 0, sizeof(*stats));

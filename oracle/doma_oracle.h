@@ -19,7 +19,11 @@
  *   - 1D sweep + SI + MG: the reference's 1D solver needs Eigen, which is absent,
  *     so it cannot be built here; 1D is pinned by the exact DD recurrence in a
  *     pure absorber, the cell balance identity, mirror symmetry of symmetric
- *     decks and the infinite-medium limit.  The 1D DSA path is not restated.
+ *     decks and the infinite-medium limit.
+ *   - 1D DSA (TransportSolver1D.cpp:587-800, needs Eigen, unbuildable here): the
+ *     tridiagonal solve against numpy.linalg.solve; the accelerated iteration
+ *     against the unaccelerated fixed point (the scheme is consistent) and its
+ *     iteration counts against the unaccelerated ones.
  *   - fission source and power iteration: 2D/3D against the reference's own
  *     compiled solveFixedSource/solveEigenvalue on the fission decks
  *     (tests/golden/decks); 1D against the dominant eigenvalue of the assembled
@@ -63,6 +67,7 @@ typedef struct oracle_problem {
   const double *prompt;
   const uint8_t *has_chi;
   const uint8_t *has_prod;
+  int dsa;                  /* 1D: diffusion synthetic acceleration after every sweep */
 } oracle_problem;
 
 typedef struct oracle_stats {
@@ -122,6 +127,19 @@ int oracle_solve_fixed_source(const oracle_problem *p, double fs_tol, int fs_max
 int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_it,
                             double mg_tol, int mg_max_it, double k_tol, int nthreads,
                             double *phi_out, double *k_eff, oracle_stats *stats);
+
+/* Tridiagonal solve by Gaussian elimination with partial pivoting (row
+ * interchanges between neighbours, as LAPACK dgtsv).  lo[i] = A[i][i-1],
+ * di[i] = A[i][i], up[i] = A[i][i+1]; the arrays are overwritten.  Used by the
+ * 1D DSA restatement; exported for its test. */
+void oracle_tridiag_solve(int n, double *lo, double *di, double *up, double *rhs, double *x);
+
+/* Alcouffe's DSA system of one 1D sweep (TransportSolver1D.cpp:587-790): phi,
+ * cur (cell currents), sfr/sfl (right/left edge fluxes of each cell) and qmg are
+ * [N]; lo/di/up/rhs [N+1] receive the tridiagonal rows.  Exported for tests. */
+void oracle_dsa_system(const oracle_problem *p, int g, int eigen, const double *phi,
+                       const double *cur, const double *sfr, const double *sfl,
+                       const double *qmg, double *lo, double *di, double *up, double *rhs);
 
 /* Source iteration of one group in isolation, starting from phi_tot[g] = 0 and
  * the given multigroup source qmg[N] (already scaled by the angular

@@ -62,6 +62,9 @@ DECKS = {
     "1D_NPRE_555_mp2": "cases/1D_NPRE_555_mp2.xml",
     "1D_NPRE_555_mp3_3g_src": "cases/1D_NPRE_555_mp3_3g_src.xml",
     "1D_NPRE_555_mp3_3g_eig": "cases/1D_NPRE_555_mp3_3g_eig.xml",
+    "1D_NPRE_555_mp3_3g_src_dsa": "cases/1D_NPRE_555_mp3_3g_src_dsa.xml",
+    "1D_NPRE_555_mp3_3g_eig_dsa": "cases/1D_NPRE_555_mp3_3g_eig_dsa.xml",
+    "1D_NPRE_555_mp1": "cases/1D_NPRE_555_mp1.xml",
     "2D_assembly_eig": "cases/2D_assembly_eig.xml",
     "2D_assembly_eig_coarse": "cases/2D_assembly_eig.xml",
     "3D_assembly_fs_coarse": "cases/3D_assembly_fs.xml",

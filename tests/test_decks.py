@@ -239,17 +239,14 @@ def test_deck_run_matches_reference(name, tmp_path):
 @pytest.mark.parametrize("name", MESH_ONLY)
 def test_deck_run_1d(name, tmp_path):
     """1D decks: mesh files against the reference's; the flux (and k) against the
-    oracle (the reference's 1D solver needs Eigen and is not built here)."""
+    oracle, with DSA where the deck asks for it (the reference's 1D solver needs
+    Eigen and is not built here)."""
     from oracle import binding as ob
 
     ref = np.load(DECKS / f"{name}.npz")
     _, p = read_deck(DECKS / f"{name}.xml")
     base = tmp_path / name
     print(f"deck {name}", flush=True)
-    if p["dsa"]:
-        with pytest.raises(doma_amd.DomaError, match="synthetic acceleration"):
-            doma_amd.run_deck(DECKS / f"{name}.xml", base)
-        return
     conv, phi, st = doma_amd.run_deck(DECKS / f"{name}.xml", base)
     for f in ("dims", "blocks", "meshx"):
         assert _sha(f"{base}_{f}.txt") == str(ref[f"sha256_{f}"]) or not conv, (name, f)
@@ -271,11 +268,12 @@ def test_deck_run_1d(name, tmp_path):
         prob.fission = capi.Fission(*f)
     if p["mode"] == "eigen":
         k, ophi, ost = ob.solve_eigenvalue(prob.normalized(), p["fs_tol"], p["fs_max_it"],
-                                           p["mg_tol"], p["mg_max_it"], p["k_tol"], nthreads=8)
+                                           p["mg_tol"], p["mg_max_it"], p["k_tol"], nthreads=8,
+                                           dsa=p["dsa"])
         assert abs(st["k_eff"] - k) <= 1e-10 * k, (st["k_eff"], k)
     else:
         ophi, ost = ob.solve_fixed_source(prob.normalized(), p["fs_tol"], p["fs_max_it"],
-                                          p["mg_tol"], p["mg_max_it"])
+                                          p["mg_tol"], p["mg_max_it"], dsa=p["dsa"])
     assert conv == ost["success"]
     assert rel_err(phi, ophi).max() < 1e-10
     assert st["si_iterations"] == ost["si_iterations"]
