@@ -24,6 +24,8 @@ SCHEME_TW_DD = 1
 FP64 = 0
 FP32_SWEEP = 1
 MAX_GROUPS = 64
+MAX_DELAYED = 8
+STEP_NONE, STEP_BOTH, STEP_INSERT, STEP_REMOVE = -1, 0, 1, 2
 
 SCATTER_DOWN1 = 0
 SCATTER_FULL = 1
@@ -115,6 +117,12 @@ SIGNATURES = [
     ("doma_solve_eigenvalue", C.c_int,
      [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_double, _dp, _dp,
       C.POINTER(Stats)]),
+    ("doma_set_kinetics", C.c_int, [C.c_void_p, C.c_int32, _dp, _ip, _dp, _dp, _dp]),
+    ("doma_set_step_sources", C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp]),
+    ("doma_transient_begin", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32]),
+    ("doma_transient_step", C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.c_double, C.c_int32,
+                                      _dp, C.POINTER(Stats)]),
+    ("doma_get_dnps", C.c_int, [C.c_void_p, _dp, _ip]),
     ("doma_get_flux", C.c_int, [C.c_void_p, _dp]),
     ("doma_time_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _ip]),
     ("doma_profile_sweeps", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _ip]),
@@ -234,6 +242,38 @@ class Fission:
 
 
 @dataclass
+class Kinetics:
+    """Transient data: inv_v [M, G]; nd [M] delayed groups; chi_d, beta [M, G, 8];
+    lambda_ [M, 8] (slots past a material's count are ignored)."""
+    inv_v: np.ndarray
+    nd: np.ndarray
+    chi_d: np.ndarray
+    beta: np.ndarray
+    lambda_: np.ndarray
+
+    def normalized(self, G: int) -> "Kinetics":
+        return Kinetics(_f64(self.inv_v).reshape(-1, G),
+                        np.ascontiguousarray(self.nd, dtype=np.int32).reshape(-1),
+                        _f64(self.chi_d).reshape(-1, G, MAX_DELAYED),
+                        _f64(self.beta).reshape(-1, G, MAX_DELAYED),
+                        _f64(self.lambda_).reshape(-1, MAX_DELAYED))
+
+
+@dataclass
+class StepSources:
+    """Per material: type [M] (STEP_*), insert_at, remove_at [M], src [M, G]."""
+    type: np.ndarray
+    insert_at: np.ndarray
+    remove_at: np.ndarray
+    src: np.ndarray
+
+    def normalized(self, G: int) -> "StepSources":
+        return StepSources(np.ascontiguousarray(self.type, dtype=np.int32).reshape(-1),
+                           _f64(self.insert_at).reshape(-1), _f64(self.remove_at).reshape(-1),
+                           _f64(self.src).reshape(-1, G))
+
+
+@dataclass
 class Problem:
     """A problem in the C-ABI layout (numpy, host); `fission` is optional."""
     dims: int
@@ -254,6 +294,8 @@ class Problem:
     name: str = ""
     meta: dict = field(default_factory=dict)
     fission: Fission | None = None
+    kinetics: Kinetics | None = None
+    steps: StepSources | None = None
 
     @property
     def nx(self):
@@ -293,7 +335,9 @@ class Problem:
             _f64(self.src).reshape(-1, self.groups),
             np.ascontiguousarray(self.has_src, dtype=np.uint8).reshape(-1),
             _f64(self.mu), _f64(self.eta), _f64(self.xi), _f64(self.w), self.name, dict(self.meta),
-            None if self.fission is None else self.fission.normalized(int(self.groups)))
+            None if self.fission is None else self.fission.normalized(int(self.groups)),
+            None if self.kinetics is None else self.kinetics.normalized(int(self.groups)),
+            None if self.steps is None else self.steps.normalized(int(self.groups)))
 
 
 class Context:
@@ -316,6 +360,16 @@ class Context:
             _check(lib().doma_set_fission(h, p.num_mats, _d(f.chi), _d(f.prod), _d(f.prompt),
                                           f.has_chi.ctypes.data_as(_u8p),
                                           f.has_prod.ctypes.data_as(_u8p)), "doma_set_fission")
+        if p.kinetics is not None:
+            kn = p.kinetics
+            _check(lib().doma_set_kinetics(h, p.num_mats, _d(kn.inv_v), kn.nd.ctypes.data_as(_ip),
+                                           _d(kn.chi_d), _d(kn.beta), _d(kn.lambda_)),
+                   "doma_set_kinetics")
+        if p.steps is not None:
+            st = p.steps
+            _check(lib().doma_set_step_sources(h, p.num_mats, st.type.ctypes.data_as(_ip),
+                                               _d(st.insert_at), _d(st.remove_at), _d(st.src)),
+                   "doma_set_step_sources")
         if tuning:
             self.set_tuning(**tuning)
 
@@ -385,6 +439,26 @@ class Context:
         if want_flux:
             phi = phi.reshape(self.problem.groups, self.problem.N)
         return k.value, phi, stats
+
+    def transient_begin(self, t0: float, t1: float, steps: int):
+        """Zero initial condition; dt = (t1 - t0) / steps; the step clock starts at 0."""
+        _check(lib().doma_transient_begin(self.h, t0, t1, steps), "doma_transient_begin")
+
+    def transient_step(self, fs_tol: float, fs_max_it: int, mg_tol: float, mg_max_it: int):
+        """Solve the next step: (t of the step, stats)."""
+        t, st = C.c_double(), Stats()
+        _check(lib().doma_transient_step(self.h, fs_tol, fs_max_it, mg_tol, mg_max_it,
+                                         C.byref(t), C.byref(st)), "doma_transient_step")
+        return t.value, _stats_dict(st, self.problem.groups)
+
+    def get_dnps(self) -> np.ndarray:
+        """Precursors [nd_max, N]."""
+        nd = C.c_int32()
+        _check(lib().doma_get_dnps(self.h, None, C.byref(nd)), "doma_get_dnps")
+        out = np.zeros((nd.value, self.problem.N))
+        if nd.value:
+            _check(lib().doma_get_dnps(self.h, _d(out), C.byref(nd)), "doma_get_dnps")
+        return out
 
     def get_flux(self) -> np.ndarray:
         phi = np.zeros(self.problem.groups * self.problem.N)

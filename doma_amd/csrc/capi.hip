@@ -106,6 +106,26 @@ iter_args(doma_ctx * c, int g)
   a.red = c->d_red;
   a.res = c->d_res;
   a.err = c->d_err;
+  if (c->transient)
+  {
+    a.inv_v = c->d_inv_v;
+    a.nd = c->d_nd;
+    a.chi_d = c->d_chi_d;
+    a.beta = c->d_beta;
+    a.lambda = c->d_lambda;
+    a.decay = c->d_decay;
+    if (c->steps_set)
+    {
+      a.step_type = c->d_step_type;
+      a.step_in = c->d_step_in;
+      a.step_out = c->d_step_out;
+      a.step_src = c->d_step_src;
+    }
+    a.last = c->d_last;
+    a.dnp = c->d_dnp;
+    a.t = c->tr_t;
+    a.dt = c->tr_dt;
+  }
   if (c->fission)
   {
     a.chi = c->d_chi;
@@ -810,6 +830,18 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_has_chi);
   dev_free(c->d_has_prod);
   dev_free(c->d_dsa);
+  dev_free(c->d_inv_v);
+  dev_free(c->d_nd);
+  dev_free(c->d_chi_d);
+  dev_free(c->d_beta);
+  dev_free(c->d_lambda);
+  dev_free(c->d_decay);
+  dev_free(c->d_step_type);
+  dev_free(c->d_step_in);
+  dev_free(c->d_step_out);
+  dev_free(c->d_step_src);
+  dev_free(c->d_last);
+  dev_free(c->d_dnp);
   dev_free(c->d_sig);
   dev_free(c->d_q);
   dev_free(c->d_qmg);
@@ -860,6 +892,7 @@ doma_set_materials(doma_ctx * c, const int32_t * mat, int32_t M, const double * 
     if (!std::isfinite(sigs[e]))
       return fail(DOMA_ERR_INVALID, "non-finite scattering cross section");
   c->fission = false;
+  c->kinetics = c->steps_set = c->transient = false;
   if (c->M != M)
   {
     dev_free(c->d_sigt);
@@ -871,6 +904,16 @@ doma_set_materials(doma_ctx * c, const int32_t * mat, int32_t M, const double * 
     dev_free(c->d_prompt);
     dev_free(c->d_has_chi);
     dev_free(c->d_has_prod);
+    dev_free(c->d_inv_v);
+    dev_free(c->d_nd);
+    dev_free(c->d_chi_d);
+    dev_free(c->d_beta);
+    dev_free(c->d_lambda);
+    dev_free(c->d_decay);
+    dev_free(c->d_step_type);
+    dev_free(c->d_step_in);
+    dev_free(c->d_step_out);
+    dev_free(c->d_step_src);
     c->M = 0;
     if ((rc = dev_alloc(&c->d_sigt, (size_t)M * G)) || (rc = dev_alloc(&c->d_sigs, (size_t)M * G * G)) ||
         (rc = dev_alloc(&c->d_src, (size_t)M * G)) || (rc = dev_alloc(&c->d_has_src, (size_t)M)))
@@ -1067,6 +1110,9 @@ doma_source_iteration(doma_ctx * c, int32_t g, const double * qmg, double tol, i
   return DOMA_OK;
 }
 
+int multigroup_loop(doma_ctx * c, double fs_tol, int fs_max_it, double mg_tol, int mg_max_it,
+                    doma_stats & s);
+
 extern "C" int
 doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_tol,
                         int32_t mg_max_it, double * phi, doma_stats * st)
@@ -1076,13 +1122,39 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
     return rc;
   if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
     return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
+  c->transient = false;  // a steady-state solve ends any transient in progress
   const double t0 = now_ms();
-  doma_stats s{};
   const size_t N = (size_t)c->N;
   // initializeSolve (TransportSolver3D.cpp:499-529): phi = 0.
   if ((rc = launch_fill(c->d_phi, N * c->G, 0.0, c->stream)) ||
       (rc = launch_fill(c->d_prev, N * c->G, 0.0, c->stream)))
     return rc;
+  doma_stats s{};
+  if ((rc = multigroup_loop(c, fs_tol, fs_max_it, mg_tol, mg_max_it, s)))
+    return rc;
+  if (phi)
+  {
+    DOMA_HIP(hipMemcpyAsync(phi, c->d_phi, sizeof(double) * N * c->G, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipGetLastError());
+  s.elapsed_ms = now_ms() - t0;
+  if (st)
+    *st = s;
+  return DOMA_OK;
+}
+
+// The multigroup Gauss-Seidel loop of solveFixedSource (TransportSolver3D.cpp:
+// 272-316) on the current flux state: per pass every group's multigroup source
+// and source iteration, then the multigroup residual; stops after mg_max_it
+// passes, at a residual <= mg_tol or with one group; success iff fewer than
+// mg_max_it passes and every source iteration converged.
+int
+multigroup_loop(doma_ctx * c, double fs_tol, int fs_max_it, double mg_tol, int mg_max_it,
+                doma_stats & s)
+{
+  int rc = DOMA_OK;
   int mg_it = 0, ok = 1;
   double r = 0.0;
   do
@@ -1122,16 +1194,6 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
   s.success = ok && mg_it < mg_max_it;
   s.mg_iterations = mg_it;
   s.mg_residual = r;
-  if (phi)
-  {
-    DOMA_HIP(hipMemcpyAsync(phi, c->d_phi, sizeof(double) * N * c->G, hipMemcpyDeviceToHost,
-                            c->stream));
-  }
-  DOMA_HIP(hipStreamSynchronize(c->stream));
-  DOMA_HIP(hipGetLastError());
-  s.elapsed_ms = now_ms() - t0;
-  if (st)
-    *st = s;
   return DOMA_OK;
 }
 
@@ -1147,6 +1209,7 @@ doma_solve_eigenvalue(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_
     return fail(DOMA_ERR_STATE, "an eigenvalue solve needs fission data (doma_set_fission)");
   if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || !(k_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
     return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
+  c->transient = false;
   const double t0 = now_ms();
   doma_stats s{};
   const size_t N = (size_t)c->N;
@@ -1222,6 +1285,204 @@ doma_solve_eigenvalue(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_
     *k_eff = k;
   if (st)
     *st = s;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_kinetics(doma_ctx * c, int32_t M, const double * inv_v, const int32_t * num_delayed,
+                  const double * chi_d, const double * beta, const double * lambda)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!inv_v || !num_delayed || !chi_d || !beta || !lambda)
+    return fail(DOMA_ERR_INVALID, "doma_set_kinetics: null array");
+  if (M != c->M)
+    return fail(DOMA_ERR_INVALID, "doma_set_kinetics: material count differs from doma_set_materials");
+  const size_t G = c->G, K = DOMA_MAX_DELAYED;
+  for (int m = 0; m < M; ++m)
+  {
+    if (num_delayed[m] < 0 || num_delayed[m] > DOMA_MAX_DELAYED)
+      return fail(DOMA_ERR_INVALID, "num_delayed must be in [0, " +
+                                        std::to_string(DOMA_MAX_DELAYED) + "]");
+    for (size_t g = 0; g < G; ++g)
+      if (!std::isfinite(inv_v[m * G + g]) || inv_v[m * G + g] < 0.0)
+        return fail(DOMA_ERR_INVALID, "inverse velocities must be finite and >= 0");
+    for (int d = 0; d < num_delayed[m]; ++d)
+    {
+      if (!(lambda[m * K + d] > 0.0) || !std::isfinite(lambda[m * K + d]))
+        return fail(DOMA_ERR_INVALID, "delayed decay constants must be finite and > 0");
+      for (size_t g = 0; g < G; ++g)
+        if (!std::isfinite(chi_d[(m * G + g) * K + d]) || !std::isfinite(beta[(m * G + g) * K + d]))
+          return fail(DOMA_ERR_INVALID, "non-finite delayed-neutron data");
+    }
+  }
+  if (!c->d_inv_v &&
+      ((rc = dev_alloc(&c->d_inv_v, (size_t)M * G)) || (rc = dev_alloc(&c->d_nd, (size_t)M)) ||
+       (rc = dev_alloc(&c->d_chi_d, (size_t)M * G * K)) || (rc = dev_alloc(&c->d_beta, (size_t)M * G * K)) ||
+       (rc = dev_alloc(&c->d_lambda, (size_t)M * K)) || (rc = dev_alloc(&c->d_decay, (size_t)M * K))))
+    return rc;
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipMemcpy(c->d_inv_v, inv_v, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_nd, num_delayed, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_chi_d, chi_d, sizeof(double) * M * G * K, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_beta, beta, sizeof(double) * M * G * K, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_lambda, lambda, sizeof(double) * M * K, hipMemcpyHostToDevice));
+  c->h_lambda.assign(lambda, lambda + (size_t)M * K);
+  c->h_nd.assign(num_delayed, num_delayed + M);
+  c->kinetics = true;
+  c->transient = false;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_step_sources(doma_ctx * c, int32_t M, const int32_t * type, const double * insert_at,
+                      const double * remove_at, const double * src)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!type || !insert_at || !remove_at || !src)
+    return fail(DOMA_ERR_INVALID, "doma_set_step_sources: null array");
+  if (M != c->M)
+    return fail(DOMA_ERR_INVALID,
+                "doma_set_step_sources: material count differs from doma_set_materials");
+  for (int m = 0; m < M; ++m)
+  {
+    if (type[m] < DOMA_STEP_NONE || type[m] > DOMA_STEP_REMOVE)
+      return fail(DOMA_ERR_INVALID, "unknown step source type");
+    if (type[m] == DOMA_STEP_BOTH && remove_at[m] < insert_at[m])
+      return fail(DOMA_ERR_INVALID, "The step source for material " + std::to_string(m) +
+                                        " is removed before it is inserted!");
+  }
+  const size_t G = c->G;
+  if (!c->d_step_type &&
+      ((rc = dev_alloc(&c->d_step_type, (size_t)M)) || (rc = dev_alloc(&c->d_step_in, (size_t)M)) ||
+       (rc = dev_alloc(&c->d_step_out, (size_t)M)) || (rc = dev_alloc(&c->d_step_src, (size_t)M * G))))
+    return rc;
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipMemcpy(c->d_step_type, type, sizeof(int32_t) * M, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_step_in, insert_at, sizeof(double) * M, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_step_out, remove_at, sizeof(double) * M, hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(c->d_step_src, src, sizeof(double) * M * G, hipMemcpyHostToDevice));
+  c->steps_set = true;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_transient_begin(doma_ctx * c, double t0, double t1, int32_t steps)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!c->kinetics)
+    return fail(DOMA_ERR_STATE, "a transient needs kinetics data (doma_set_kinetics)");
+  if (steps < 1 || !std::isfinite(t0) || !std::isfinite(t1) || !(t1 > t0))
+    return fail(DOMA_ERR_INVALID, "a transient needs t1 > t0 and steps >= 1");
+  for (int m = 0; m < c->M; ++m)
+    if (c->h_nd[m] > 0 && !c->fission)
+      return fail(DOMA_ERR_STATE, "materials with delayed groups need nu_sigma_f (doma_set_fission)");
+  if (c->fission)
+  {
+    std::vector<uint8_t> has_prod(c->M);
+    DOMA_HIP(hipMemcpy(has_prod.data(), c->d_has_prod, (size_t)c->M, hipMemcpyDeviceToHost));
+    for (int m = 0; m < c->M; ++m)
+      if (c->h_nd[m] > 0 && !has_prod[m])
+        return fail(DOMA_ERR_INVALID, "material " + std::to_string(m) +
+                                          " has delayed groups but no nu_sigma_f");
+  }
+  const size_t N = (size_t)c->N, K = DOMA_MAX_DELAYED;
+  if (!c->d_last && ((rc = dev_alloc(&c->d_last, N * c->G)) || (rc = dev_alloc(&c->d_dnp, N * K))))
+    return rc;
+  // dt and exp(-lambda dt) on the host, as the reference computes them
+  // (TransportSolver3D.cpp:21, 255).
+  c->tr_dt = (t1 - t0) / static_cast<double>(steps);
+  std::vector<double> decay((size_t)c->M * K, 0.0);
+  for (int m = 0; m < c->M; ++m)
+    for (int d = 0; d < c->h_nd[m]; ++d)
+      decay[m * K + d] = std::exp(-c->h_lambda[m * K + d] * c->tr_dt);
+  DOMA_HIP(hipMemcpy(c->d_decay, decay.data(), sizeof(double) * decay.size(), hipMemcpyHostToDevice));
+  // initZeroIC (TransportSolver3D.cpp:140-156).
+  if ((rc = launch_fill(c->d_phi, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_prev, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_last, N * c->G, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_dnp, N * K, 0.0, c->stream)) ||
+      (rc = launch_fill(c->d_q, N, 0.0, c->stream)))
+    return rc;
+  c->tr_t = 0.0;
+  c->tr_step = 0;
+  c->tr_steps = steps;
+  c->transient = true;
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_transient_step(doma_ctx * c, double fs_tol, int32_t fs_max_it, double mg_tol,
+                    int32_t mg_max_it, double * t_solved, doma_stats * st)
+{
+  int rc = check_ctx(c, true);
+  if (rc)
+    return rc;
+  if (!c->transient)
+    return fail(DOMA_ERR_STATE, "doma_transient_begin has not been called");
+  if (c->tr_step >= c->tr_steps)
+    return fail(DOMA_ERR_STATE, "all " + std::to_string(c->tr_steps) + " steps have been solved");
+  if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
+    return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
+  const double t0 = now_ms();
+  const size_t N = (size_t)c->N;
+  // updateStepFluxes between steps (TransportSolver3D.cpp:204-228): the last
+  // step's flux becomes phi_last and phi restarts from zero.
+  if (c->tr_step > 0)
+  {
+    DOMA_HIP(hipMemcpyAsync(c->d_last, c->d_phi, sizeof(double) * N * c->G,
+                            hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = launch_fill(c->d_phi, N * c->G, 0.0, c->stream)))
+      return rc;
+  }
+  doma_stats s{};
+  if ((rc = multigroup_loop(c, fs_tol, fs_max_it, mg_tol, mg_max_it, s)))
+    return rc;
+  if (s.success)
+  {
+    IterArgs a = iter_args(c, 0);
+    if ((rc = launch_step_dnps(a, c->stream)))
+      return rc;
+  }
+  DOMA_HIP(hipStreamSynchronize(c->stream));
+  DOMA_HIP(hipGetLastError());
+  if (t_solved)
+    *t_solved = c->tr_t;
+  if (s.success)
+  {
+    c->tr_t += c->tr_dt;
+    ++c->tr_step;
+  }
+  s.elapsed_ms = now_ms() - t0;
+  if (st)
+    *st = s;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_get_dnps(doma_ctx * c, double * dnps, int32_t * nd_max)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  int nd = 0;
+  for (int v : c->h_nd)
+    nd = std::max(nd, v);
+  if (nd_max)
+    *nd_max = nd;
+  if (!dnps || nd == 0)
+    return DOMA_OK;
+  if (!c->d_dnp)
+    return fail(DOMA_ERR_STATE, "no transient has been started");
+  DOMA_HIP(hipMemcpyAsync(dnps, c->d_dnp, sizeof(double) * (size_t)nd * c->N,
+                          hipMemcpyDeviceToHost, c->stream));
+  DOMA_HIP(hipStreamSynchronize(c->stream));
   return DOMA_OK;
 }
 

@@ -153,6 +153,21 @@ struct IterArgs
   int eigen;               // 1: power-iteration source (fission / k, no external source)
   double k;                // current eigenvalue (eigen != 0)
   const double * dsa;      // [N] 1D: accelerated flux replacing the swept one (or null)
+  // Transient step (null inv_v: steady state).
+  const double * inv_v;    // [M][G]
+  const int32_t * nd;      // [M] delayed groups per material
+  const double * chi_d;    // [M][G][DOMA_MAX_DELAYED]
+  const double * beta;     // [M][G][DOMA_MAX_DELAYED]
+  const double * lambda;   // [M][DOMA_MAX_DELAYED]
+  const double * decay;    // [M][DOMA_MAX_DELAYED] exp(-lambda dt)
+  const int32_t * step_type;  // [M] DOMA_STEP_* (null: no step sources)
+  const double * step_in;  // [M]
+  const double * step_out; // [M]
+  const double * step_src; // [M][G]
+  const double * last;     // [G][N] flux of the previous step
+  double * dnp;            // [DOMA_MAX_DELAYED][N] precursor concentrations
+  double t;                // time of the step
+  double dt;
 };
 
 // 1D diffusion synthetic acceleration of one sweep (see iterate.hip).
@@ -178,6 +193,8 @@ int launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s);
 // 1D DSA after a sweep: moments from the direction partials, the tridiagonal
 // diffusion system, its solve, the cell-average flux into d.out.
 int launch_dsa1d(const IterArgs & a, const DsaArgs & d, hipStream_t s);
+// stepDNPs (TransportSolver3D.cpp:230-260): precursor update after a step.
+int launch_step_dnps(const IterArgs & a, hipStream_t s);
 int launch_fill(double * p, size_t n, double v, hipStream_t s);
 // Batched finish/residual over problem slots (see iterate.hip), training-data
 // snapshots and targets.
@@ -226,6 +243,24 @@ struct doma_ctx
   uint8_t * d_has_src = nullptr;
   bool fission = false;  // doma_set_fission data present for the current materials
   int dsa = 0;            // 1D diffusion synthetic acceleration (doma_set_dsa)
+  // Transient state (doma_set_kinetics, doma_set_step_sources, doma_transient_*).
+  bool kinetics = false, steps_set = false, transient = false;
+  double * d_inv_v = nullptr;
+  int32_t * d_nd = nullptr;
+  double * d_chi_d = nullptr;
+  double * d_beta = nullptr;
+  double * d_lambda = nullptr;
+  double * d_decay = nullptr;
+  int32_t * d_step_type = nullptr;
+  double * d_step_in = nullptr;
+  double * d_step_out = nullptr;
+  double * d_step_src = nullptr;
+  double * d_last = nullptr;  // [G][N]
+  double * d_dnp = nullptr;   // [DOMA_MAX_DELAYED][N]
+  std::vector<double> h_lambda;  // [M][DOMA_MAX_DELAYED]
+  std::vector<int32_t> h_nd;
+  double tr_t = 0.0, tr_dt = 0.0;
+  int tr_step = 0, tr_steps = 0;
   int solving_eigen = 0;  // inside doma_solve_eigenvalue (selects the DSA scheme)
   double dsa_beta[2] = {0.0, 0.0};
   double * d_dsa = nullptr;  // DSA work [4 N + 6 (N + 1)] + output [N]

@@ -111,12 +111,23 @@ mg_source_kernel(IterArgs a)
   const int m = a.mat[c];
   const int G = a.G, g = a.g;
   const bool fis = a.chi && a.has_chi[m];
+  const bool tr = a.inv_v != nullptr;
   double acc;
   if (a.eigen)
     acc = a.dims == 1 ? 0.0 : a.q[c];
   else
     acc = a.has_src[m] ? norm_src(a.dims, a.src[(size_t)m * G + g]) : 0.0;
+  // Step source of the cell's block at the step's time (TransportSolver3D.cpp:353-369).
+  if (tr && a.step_type && a.step_type[m] != DOMA_STEP_NONE)
+  {
+    const int ty = a.step_type[m];
+    const double tin = a.step_in[m], tout = a.step_out[m];
+    const bool on = ty == DOMA_STEP_BOTH ? (tin <= a.t && a.t <= tout)
+                                         : (ty == DOMA_STEP_INSERT ? tin <= a.t : a.t <= tout);
+    acc += on ? norm_src(a.dims, a.step_src[(size_t)m * G + g]) : 0.0;
+  }
   const double chi = fis ? a.chi[(size_t)m * G + g] : 0.0;
+  const int nd = tr ? a.nd[m] : 0;
   for (int gp = 0; gp < G; ++gp)
   {
     const double f = a.phi[(size_t)gp * a.N + c];
@@ -130,6 +141,22 @@ mg_source_kernel(IterArgs a)
       else
         acc += fission_term(a.dims, a.prompt[(size_t)m * G + gp], chi, pr, f);
     }
+    // The reference adds the delayed and the time-derivative sources inside the
+    // loop over source groups, i.e. G times each (TransportSolver3D.cpp:388-396).
+    for (int d = 0; d < nd; ++d)
+    {
+      double t = (a.dims == 3 ? 0.25 : 0.5) * a.chi_d[((size_t)m * G + g) * DOMA_MAX_DELAYED + d];
+      t = t * a.dnp[(size_t)d * a.N + c];
+      t = t * a.lambda[(size_t)m * DOMA_MAX_DELAYED + d];
+      acc += a.dims == 1 ? t : t / PI;
+    }
+    if (tr)
+    {
+      double t = (a.dims == 3 ? 0.25 : 0.5) * a.last[(size_t)g * a.N + c];
+      t = t * a.inv_v[(size_t)m * G + g];
+      t = t / a.dt;
+      acc += a.dims == 1 ? t : t / PI;
+    }
   }
   if (a.dims == 1)
     a.qmg[c] = acc;
@@ -137,7 +164,35 @@ mg_source_kernel(IterArgs a)
     a.q[c] = acc;
   a.prev[(size_t)g * a.N + c] = a.phi[(size_t)g * a.N + c];
   a.phi[(size_t)g * a.N + c] = 0.0;
-  a.sig[c] = a.sigt[(size_t)m * G + g];
+  // Transient: the cell solves use sigma_t + inv_v / dt (DiamondDifference3D.h:25-26).
+  a.sig[c] = tr ? a.sigt[(size_t)m * G + g] + a.inv_v[(size_t)m * G + g] / a.dt
+                : a.sigt[(size_t)m * G + g];
+}
+
+// stepDNPs (TransportSolver3D.cpp:230-260): with the step's flux, per delayed
+// group S = sum_g nuSf_g phi_g beta_gd and C = S / lambda (1 - e^{-lambda dt}) +
+// C e^{-lambda dt} (e^{-lambda dt} precomputed on the host with the C library's
+// exp); groups past the material's count stay zero.
+__global__ void
+step_dnps_kernel(IterArgs a)
+{
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N)
+    return;
+  const int m = a.mat[c];
+  const int nd = a.nd[m];
+  for (int d = 0; d < nd; ++d)
+  {
+    double src = 0.0;
+    for (int g = 0; g < a.G; ++g)
+      src += a.prod[(size_t)m * a.G + g] * a.phi[(size_t)g * a.N + c] *
+             a.beta[((size_t)m * a.G + g) * DOMA_MAX_DELAYED + d];
+    const double lam = a.lambda[(size_t)m * DOMA_MAX_DELAYED + d];
+    const double e = a.decay[(size_t)m * DOMA_MAX_DELAYED + d];
+    const size_t k = (size_t)d * a.N + c;
+    a.dnp[k] = (src / lam) * (1.0 - e) + a.dnp[k] * e;
+  }
 }
 
 // Sum of the direction-set partials, the scattering residual terms and the
@@ -704,6 +759,13 @@ launch_fission_rates(const IterArgs & a, int nblocks, hipStream_t s)
 {
   hipLaunchKernelGGL(fission_rate_kernel, dim3(nblocks), dim3(RED_THREADS), 0, s, a);
   hipLaunchKernelGGL(rate_final_kernel, dim3(1), dim3(RED_THREADS), 0, s, a, nblocks);
+  return DOMA_OK;
+}
+
+int
+launch_step_dnps(const IterArgs & a, hipStream_t s)
+{
+  hipLaunchKernelGGL(step_dnps_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
   return DOMA_OK;
 }
 

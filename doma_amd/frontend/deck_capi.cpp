@@ -89,7 +89,9 @@ run_solver(Solver & s, Mesh & m, RunMode mode, const char * output_base, int32_t
     s.options().tuning = *tuning;
   s.options().autotune = autotune != 0;
   const std::string base = output_base ? output_base : "";
-  const bool ok = mode == RunMode::Eigen ? s.solveEigenvalue(base) : s.solveFixedSource(base);
+  const bool ok = mode == RunMode::Eigen       ? s.solveEigenvalue(base)
+                  : mode == RunMode::Transient ? s.solveTransient(base)
+                                               : s.solveFixedSource(base);
   if (!s.lastError().empty())
     return fail(DOMA_ERR_INVALID, s.lastError());
   if (converged)
@@ -116,8 +118,6 @@ doma_run_deck(const char * path, const char * output_base, int32_t precision,
   try
   {
     const InputParameters p = parseInputParameters(path);
-    if (p._mode == RunMode::Transient)
-      return fail(DOMA_ERR_INVALID, "transient decks (solve_type=\"transient\") are not supported");
     if (p._num_dims == 1)
     {
       if (p._eq_type == EquationType::TW_DD)

@@ -1,6 +1,6 @@
 // doma_main.cpp -- command-line driver with the reference's interface
 // (src/main.cpp:16-298): doma -i deck.xml [--verbose] [-n threads], solving
-// fixed-source and eigenvalue decks on the GPU and writing
+// fixed-source, eigenvalue and transient decks on the GPU and writing
 // <deck dir>/<deck stem>_*.txt.
 // Extra options of this build: --fp32 (FP32-sweep mode), --autotune.
 #include <chrono>
@@ -22,8 +22,8 @@ usage(const char * argv0)
 {
   std::cerr << "Usage: " << argv0
             << " -i DECK.xml [--verbose] [-n N_THREADS] [--fp32] [--autotune]\n"
-               "Discrete Ordinates Mini-App (MI355X build): fixed-source and k-eigenvalue S_N "
-               "solves on the GPU.\n";
+               "Discrete Ordinates Mini-App (MI355X build): fixed-source, k-eigenvalue and "
+               "transient S_N solves on the GPU.\n";
 }
 
 template <typename S>
@@ -32,7 +32,9 @@ solve(S & s, RunMode mode, bool fp32, bool autotune, const std::string & base)
 {
   s.options().precision = fp32 ? DOMA_FP32_SWEEP : DOMA_FP64;
   s.options().autotune = autotune;
-  return mode == RunMode::Eigen ? s.solveEigenvalue(base) : s.solveFixedSource(base);
+  return mode == RunMode::Eigen       ? s.solveEigenvalue(base)
+         : mode == RunMode::Transient ? s.solveTransient(base)
+                                      : s.solveFixedSource(base);
 }
 } // namespace
 
@@ -81,13 +83,6 @@ main(int argc, char ** argv)
   try
   {
     const InputParameters params = parseInputParameters(inp.string());
-    if (params._mode == RunMode::Transient)
-    {
-      std::cerr << "Transient mode is outside this build's scope (solve_type=\"steady\" or "
-                   "\"eigen\")."
-                << std::endl;
-      return 1;
-    }
     bool ok = false;
     if (params._num_dims == 1)
     {
@@ -144,8 +139,9 @@ main(int argc, char ** argv)
     }
     if (!ok)
       return 2;
-    std::cout << (params._mode == RunMode::Eigen ? "Eigenvalue solver finished executing."
-                                                 : "Fixed source solver finished executing.")
+    std::cout << (params._mode == RunMode::Eigen       ? "Eigenvalue solver finished executing."
+                  : params._mode == RunMode::Transient ? "Transient solver finished executing."
+                                                       : "Fixed source solver finished executing.")
               << std::endl;
     const double s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();

@@ -4,6 +4,7 @@
 // arguments are accepted and must be vacuum.
 #include "mesh.h"
 
+#include <algorithm>
 #include <fstream>
 #include <iomanip>
 
@@ -186,6 +187,33 @@ BrickMesh::dumpToTextFile(const std::string & file_name, bool only_flux) const
     flux << std::setprecision(6);
     for (unsigned int c = 0; c < _num_cells; ++c)
       flux << _phi[(size_t)g * _num_cells + c] << std::endl;
+  }
+}
+
+unsigned int
+BrickMesh::maxDelayedGroups() const
+{
+  unsigned int nd = 0;
+  for (unsigned int c = 0; c < _num_cells; ++c)
+    nd = std::max(nd, _params._block_mat_info.at(_block[c])._num_d_groups);
+  return nd;
+}
+
+void
+BrickMesh::dumpDNPsToTextFile(const std::string & file_name, const std::vector<double> & dnps,
+                              unsigned int nd) const
+{
+  for (unsigned int d = 0; d < nd; ++d)
+  {
+    std::ofstream out(file_name + "_d" + std::to_string(d) + "_dnps.txt");
+    out << std::setprecision(6);
+    for (unsigned int c = 0; c < _num_cells; ++c)
+    {
+      if (_params._block_mat_info.at(_block[c])._num_d_groups == nd)
+        out << dnps[(size_t)d * _num_cells + c] << std::endl;
+      else
+        out << 0.0 << std::endl;
+    }
   }
 }
 

@@ -41,6 +41,13 @@ public:
 
   // Text dumps readable by the reference's plotting scripts.
   void dumpToTextFile(const std::string & file_name, bool only_flux = false) const;
+  // Precursor dumps <file_name>_d<d>_dnps.txt (BrickMesh3D::dumpDNPsToTextFile):
+  // dnps [nd][N]; a cell whose material has other than the mesh's largest number
+  // of delayed groups prints 0.
+  void dumpDNPsToTextFile(const std::string & file_name, const std::vector<double> & dnps,
+                          unsigned int nd) const;
+  // Largest delayed-group count over the materials of the mesh's cells.
+  unsigned int maxDelayedGroups() const;
 
 protected:
   BrickMesh(const InputParameters & params, unsigned int dims);

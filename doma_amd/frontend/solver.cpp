@@ -5,6 +5,7 @@
 // TransportSolver1D.cpp:12).  Console lines mirror the reference's summaries.
 #include "solver.h"
 
+#include <algorithm>
 #include <iostream>
 #include <map>
 
@@ -36,8 +37,9 @@ SolverCore::fixedSourceOnly()
 bool
 SolverCore::run(const std::string & output_file_base, int dims, int dirs,
                 const std::vector<double> & mu, const std::vector<double> & eta,
-                const std::vector<double> & xi, const std::vector<double> & w, bool eigen)
+                const std::vector<double> & xi, const std::vector<double> & w, Mode mode)
 {
+  const bool eigen = mode == Mode::Eigen;
   _error.clear();
   const unsigned int G = _params._num_e_groups;
   if (G < 1 || G > DOMA_MAX_GROUPS)
@@ -153,12 +155,70 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
                           has_prod.data());
   if (rc == DOMA_OK && _use_dsa)
     rc = doma_set_dsa(ctx, 1);
+  if (rc == DOMA_OK && mode == Mode::Transient)
+  {
+    // Kinetics data and step sources (MaterialProps, SourceStep) by material index.
+    const size_t K = DOMA_MAX_DELAYED;
+    std::vector<double> inv_v((size_t)M * G, 0.0), chi_d((size_t)M * G * K, 0.0),
+        beta((size_t)M * G * K, 0.0), lambda((size_t)M * K, 0.0), step_in(M, 0.0),
+        step_out(M, 0.0), step_src((size_t)M * G, 0.0);
+    std::vector<int32_t> nd(M, 0), step_type(M, DOMA_STEP_NONE);
+    for (const auto & kv : index)
+    {
+      const MaterialProps & p = _params._block_mat_info.at(kv.first);
+      const int mi = kv.second;
+      if (p._g_inv_v.size() != G || p._num_d_groups > DOMA_MAX_DELAYED)
+      {
+        _error = "Block " + std::to_string(kv.first) +
+                 " needs inverse velocities and at most " + std::to_string(DOMA_MAX_DELAYED) +
+                 " delayed groups for a transient";
+        std::cerr << _error << std::endl;
+        doma_destroy(ctx);
+        return false;
+      }
+      nd[mi] = (int32_t)p._num_d_groups;
+      for (unsigned int g = 0; g < G; ++g)
+      {
+        inv_v[(size_t)mi * G + g] = p._g_inv_v[g];
+        for (unsigned int d = 0; d < p._num_d_groups; ++d)
+        {
+          chi_d[((size_t)mi * G + g) * K + d] = p._n_g_chi_d[g * p._num_d_groups + d];
+          beta[((size_t)mi * G + g) * K + d] = p._g_n_beta[g * p._num_d_groups + d];
+        }
+      }
+      for (unsigned int d = 0; d < p._num_d_groups; ++d)
+        lambda[(size_t)mi * K + d] = p._n_lambda[d];
+      auto st = _params._block_step_src.find(kv.first);
+      if (st != _params._block_step_src.end())
+      {
+        step_type[mi] = st->second._type == StepType::Both
+                            ? DOMA_STEP_BOTH
+                            : (st->second._type == StepType::Insert ? DOMA_STEP_INSERT
+                                                                    : DOMA_STEP_REMOVE);
+        step_in[mi] = st->second._insert_time;
+        step_out[mi] = st->second._remove_time;
+        for (unsigned int g = 0; g < G && g < st->second._g_src.size(); ++g)
+          step_src[(size_t)mi * G + g] = st->second._g_src[g];
+      }
+    }
+    rc = doma_set_kinetics(ctx, M, inv_v.data(), nd.data(), chi_d.data(), beta.data(),
+                           lambda.data());
+    if (rc == DOMA_OK && !_params._block_step_src.empty())
+      rc = doma_set_step_sources(ctx, M, step_type.data(), step_in.data(), step_out.data(),
+                                 step_src.data());
+  }
   if (rc == DOMA_OK && (_opt.tuning.brick || _opt.tuning.threads || _opt.tuning.exact_arith ||
                         _opt.tuning.schedule || _opt.tuning.octants || _opt.tuning.dirs_per_lane ||
                         _opt.tuning.shape))
     rc = doma_set_tuning(ctx, &_opt.tuning);
   if (rc == DOMA_OK && _opt.autotune)
     rc = doma_autotune(ctx, 0, nullptr, nullptr);
+  if (rc == DOMA_OK && mode == Mode::Transient)
+  {
+    const bool ok = transientSteps(ctx, output_file_base);
+    doma_destroy(ctx);
+    return ok;
+  }
   if (rc == DOMA_OK && eigen)
     rc = doma_solve_eigenvalue(ctx, _params._src_it_tol, (int)_params._num_src_it, _params._gs_tol,
                                (int)_params._num_mg_it, _params._k_tol, _mesh.flux().data(),
@@ -218,6 +278,81 @@ SolverCore::run(const std::string & output_file_base, int dims, int dirs,
   return _stats.success != 0;
 }
 
+// solveTransient (TransportSolver3D.cpp:88-135) on a prepared context: zero
+// initial condition, its dump, then per step the solve, the flux and precursor
+// dumps at <base>_t<step>.
+bool
+SolverCore::transientSteps(doma_ctx * ctx, const std::string & output_file_base)
+{
+  const unsigned int G = _params._num_e_groups;
+  std::cout << "Setting up initial conditions..." << std::endl;
+  if (_params._ic != TransientIC::Zero)
+  {
+    // initSteadyIC runs solveFixedSource in transient mode before the previous-
+    // step flux and precursor vectors are sized (TransportSolver2D.cpp:158-200
+    // with :377-396), so its result is undefined; only ics="zero" is solved.
+    return refuse("Transient mode supports ics=\"zero\" only: the reference's steady-state "
+                  "initial condition reads unsized vectors.");
+  }
+  int rc = doma_transient_begin(ctx, _params._t0, _params._t1, (int)_params._num_steps);
+  if (rc != DOMA_OK)
+  {
+    _error = doma_last_error();
+    std::cerr << "doma: " << _error << std::endl;
+    return false;
+  }
+  std::fill(_mesh.flux().begin(), _mesh.flux().end(), 0.0);
+  if (!output_file_base.empty())
+    _mesh.dumpToTextFile(output_file_base);
+  const unsigned int nd = _mesh.maxDelayedGroups();
+  int32_t nd_all = 0;
+  doma_get_dnps(ctx, nullptr, &nd_all);
+  std::vector<double> dnps((size_t)std::max<int32_t>(nd_all, 1) * _mesh.numCells());
+  for (unsigned int step = 0; step < _params._num_steps; ++step)
+  {
+    std::cout << "Solving timestep " << step << "..." << std::endl;
+    double t = 0.0;
+    rc = doma_transient_step(ctx, _params._src_it_tol, (int)_params._num_src_it, _params._gs_tol,
+                             (int)_params._num_mg_it, &t, &_stats);
+    if (rc == DOMA_OK)
+      rc = doma_get_flux(ctx, _mesh.flux().data());
+    if (rc == DOMA_OK && nd_all > 0)
+      rc = doma_get_dnps(ctx, dnps.data(), &nd_all);
+    if (rc != DOMA_OK)
+    {
+      _error = doma_last_error();
+      std::cerr << "doma: " << _error << std::endl;
+      return false;
+    }
+    for (unsigned int g = 0; g < G && g < DOMA_MAX_GROUPS; ++g)
+    {
+      if (_stats.si_iterations[g] == 0)
+        break;
+      const bool conv = _stats.si_iterations[g] < (int)_params._num_src_it;
+      std::cout << (conv ? "SI converged after " : "SI failed to converge after ")
+                << _stats.si_iterations[g] << " iterations with a residual of "
+                << _stats.si_residual[g] << " for G" << g << std::endl;
+    }
+    if (!_stats.success)
+    {
+      if (_stats.mg_iterations >= (int)_params._num_mg_it)
+        std::cout << "MGI failed to converge after " << _stats.mg_iterations
+                  << " iterations with a residual of " << _stats.mg_residual << std::endl;
+      return false;
+    }
+    if (G > 1)
+      std::cout << "MGI converged after " << _stats.mg_iterations << " iterations with a residual of "
+                << _stats.mg_residual << std::endl;
+    if (!output_file_base.empty())
+    {
+      const std::string b = output_file_base + "_t" + std::to_string(step);
+      _mesh.dumpToTextFile(b, true);
+      _mesh.dumpDNPsToTextFile(b, dnps, nd);
+    }
+  }
+  return true;
+}
+
 namespace
 {
 // Flatten a product quadrature into [octant slot][n] arrays; 2D keeps the four
@@ -273,14 +408,17 @@ TransportSolver3D<T>::solveEigenvalue(const std::string & output_file_base)
   _error.clear();
   std::vector<double> mu, eta, xi, w;
   const int per = flatten(_angular_quad, 3, mu, eta, xi, w);
-  return run(output_file_base, 3, per, mu, eta, xi, w, true);
+  return run(output_file_base, 3, per, mu, eta, xi, w, Mode::Eigen);
 }
 
 template <typename T>
 bool
-TransportSolver3D<T>::solveTransient(const std::string &)
+TransportSolver3D<T>::solveTransient(const std::string & output_file_base)
 {
-  return refuse("Transient mode is outside this build's scope.");
+  _error.clear();
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 3, mu, eta, xi, w);
+  return run(output_file_base, 3, per, mu, eta, xi, w, Mode::Transient);
 }
 
 // ---------------------------------------------------------------- 2D ------- //
@@ -311,14 +449,17 @@ TransportSolver2D<T>::solveEigenvalue(const std::string & output_file_base)
   _error.clear();
   std::vector<double> mu, eta, xi, w;
   const int per = flatten(_angular_quad, 2, mu, eta, xi, w);
-  return run(output_file_base, 2, per, mu, eta, xi, w, true);
+  return run(output_file_base, 2, per, mu, eta, xi, w, Mode::Eigen);
 }
 
 template <typename T>
 bool
-TransportSolver2D<T>::solveTransient(const std::string &)
+TransportSolver2D<T>::solveTransient(const std::string & output_file_base)
 {
-  return refuse("Transient mode is outside this build's scope.");
+  _error.clear();
+  std::vector<double> mu, eta, xi, w;
+  const int per = flatten(_angular_quad, 2, mu, eta, xi, w);
+  return run(output_file_base, 2, per, mu, eta, xi, w, Mode::Transient);
 }
 
 // ---------------------------------------------------------------- 1D ------- //
@@ -359,14 +500,22 @@ TransportSolver1D<T>::solveEigenvalue(const std::string & output_file_base)
     mu[i] = _angular_quad.direction(i);
     w[i] = _angular_quad.weight(i);
   }
-  return run(output_file_base, 1, n / 2, mu, eta, xi, w, true);
+  return run(output_file_base, 1, n / 2, mu, eta, xi, w, Mode::Eigen);
 }
 
 template <typename T>
 bool
-TransportSolver1D<T>::solveTransient(const std::string &)
+TransportSolver1D<T>::solveTransient(const std::string & output_file_base)
 {
-  return refuse("Transient mode is outside this build's scope.");
+  _error.clear();
+  const int n = (int)_angular_quad.order();
+  std::vector<double> mu(n), eta(n, 0.0), xi(n, 0.0), w(n);
+  for (int i = 0; i < n; ++i)
+  {
+    mu[i] = _angular_quad.direction(i);
+    w[i] = _angular_quad.weight(i);
+  }
+  return run(output_file_base, 1, n / 2, mu, eta, xi, w, Mode::Transient);
 }
 
 template class TransportSolver3D<TWDiamondDifference3D>;

@@ -40,10 +40,16 @@ public:
 
 protected:
   SolverCore(BrickMesh & mesh, const InputParameters & params, bool verbose, int scheme);
-  // eigen = false: solveFixedSource; true: solveEigenvalue (power iteration).
+  enum class Mode
+  {
+    FixedSource,  // solveFixedSource
+    Eigen,        // solveEigenvalue (power iteration)
+    Transient     // solveTransient (zero initial condition)
+  };
   bool run(const std::string & output_file_base, int dims, int dirs, const std::vector<double> & mu,
            const std::vector<double> & eta, const std::vector<double> & xi,
-           const std::vector<double> & w, bool eigen = false);
+           const std::vector<double> & w, Mode mode = Mode::FixedSource);
+  bool transientSteps(doma_ctx * ctx, const std::string & output_file_base);
   bool fixedSourceOnly();
   bool refuse(const std::string & why);
 
@@ -67,7 +73,8 @@ public:
   bool solveFixedSource(const std::string & output_file_base = "", const double & t = 0.0);
   // k-eigenvalue power iteration (TransportSolver3D.cpp:29-86).
   bool solveEigenvalue(const std::string & output_file_base = "");
-  // Transient mode is outside this build's scope.
+  // Transient from a zero initial condition (TransportSolver3D.cpp:88-135); the
+  // steady-state initial condition is refused (see solver.cpp).
   bool solveTransient(const std::string & output_file_base = "");
   const GCAngularQuadrature & quadrature() const { return _angular_quad; }
 

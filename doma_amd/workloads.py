@@ -18,6 +18,9 @@ seeded-media generator (no oracle involved).
   log-uniform in [0.5, 10].
 * ``batched_uneven`` -- independent seeded problems of uneven size for the
   batched path.
+* ``kinetic_blocks`` -- ``fissile_blocks`` (subcritical) with transient data:
+  inverse velocities per group, two delayed groups on the fissile materials,
+  step sources of every type on the others.
 * ``fissile_blocks`` -- ``random_blocks`` with fission data on a seeded share of
   the blocks: chi_p concentrated in the fastest groups, nu*sigma_f a seeded
   multiple of each group's absorption; below critical for fixed-source solves
@@ -185,4 +188,39 @@ def fissile_blocks(n: int = 32, groups: int = 3, np_: int = 2, na: int = 2,
         p.has_src = np.ones(p.num_mats, dtype=np.uint8)
     p.fission = fission_for(p, seed, nu_ratio=nu_ratio)
     p.name = f"fissile_blocks_{dims}d_{n}c_{groups}g"
+    return p
+
+
+def kinetic_blocks(n: int = 16, groups: int = 2, np_: int = 2, na: int = 2,
+                   scheme: int = capi.SCHEME_TW_DD, seed: int = SEED_BASE + 7, dims: int = 3,
+                   dt: float = 0.01) -> Problem:
+    """Transient workload: inv_v_g = 10^(-9 + 2g) s/cm (fast to thermal), two
+    delayed groups (beta 0.002/0.004 per source group, lambda 0.08/1.2 /s, chi_d
+    in the second-fastest group) on fissile materials, prompt fraction 1 - sum
+    beta; non-fissile materials carry step sources cycling through both/insert/
+    remove with times on the step grid of width dt."""
+    p = fissile_blocks(n, groups, np_, na, scheme, seed, nu_ratio=(0.3, 0.6), dims=dims)
+    M, G, K = p.num_mats, groups, capi.MAX_DELAYED
+    f = p.fission
+    inv_v = np.tile(10.0 ** (-9.0 + 2.0 * np.arange(G)), (M, 1))
+    nd = np.where(f.has_chi > 0, 2, 0).astype(np.int32)
+    chi_d, beta, lam = np.zeros((M, G, K)), np.zeros((M, G, K)), np.zeros((M, K))
+    for m in range(M):
+        if nd[m]:
+            chi_d[m, min(1, G - 1), :2] = 1.0
+            beta[m, :, 0], beta[m, :, 1] = 0.002, 0.004
+            lam[m, :2] = (0.08, 1.2)
+    prompt = np.ones((M, G))
+    prompt[nd > 0] = 1.0 - (0.002 + 0.004)
+    p.fission = capi.Fission(f.chi, f.prod, prompt, f.has_chi, f.has_prod)
+    p.kinetics = capi.Kinetics(inv_v, nd, chi_d, beta, lam)
+    ty = np.full(M, capi.STEP_NONE, dtype=np.int32)
+    t_in, t_out, s_src = np.zeros(M), np.zeros(M), np.zeros((M, G))
+    others = [m for m in range(M) if nd[m] == 0]
+    for j, m in enumerate(others):
+        ty[m] = (capi.STEP_BOTH, capi.STEP_INSERT, capi.STEP_REMOVE)[j % 3]
+        t_in[m], t_out[m] = dt, 2 * dt
+        s_src[m, 0] = 10.0
+    p.steps = capi.StepSources(ty, t_in, t_out, s_src)
+    p.name = f"kinetic_blocks_{dims}d_{n}c_{G}g"
     return p

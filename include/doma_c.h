@@ -20,6 +20,8 @@
  *     src/TransportSolver3D.cpp:262-402, 499-529)       -> doma_solve_fixed_source
  *   - TransportSolver3D<T>::solveEigenvalue (src/TransportSolver3D.cpp:29-86,
  *     404-456)                                          -> doma_solve_eigenvalue
+ *   - TransportSolver3D<T>::solveTransient (src/TransportSolver3D.cpp:88-260)
+ *                                  -> doma_transient_begin / doma_transient_step
  *   - the 2D and 1D counterparts (src/TransportSolver2D.cpp:262-725,
  *     src/TransportSolver1D.cpp:292-586, 802-915)       -> same entry points, dims 2/1
  */
@@ -45,6 +47,12 @@ extern "C" {
                                 accumulation, sources and residuals */
 
 #define DOMA_MAX_GROUPS 64
+#define DOMA_MAX_DELAYED 8   /* delayed-neutron precursor groups per material */
+
+#define DOMA_STEP_NONE (-1)  /* step source types (SourceStep, include/InputParameters.h:10-21) */
+#define DOMA_STEP_BOTH 0     /* on for insert_at <= t <= remove_at */
+#define DOMA_STEP_INSERT 1   /* on for insert_at <= t */
+#define DOMA_STEP_REMOVE 2   /* on for t <= remove_at */
 
 typedef struct doma_ctx doma_ctx;
 
@@ -169,6 +177,41 @@ int doma_solve_fixed_source(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, dou
 int doma_solve_eigenvalue(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, double mg_tol,
                           int32_t mg_max_it, double k_tol, double *phi, double *k_eff,
                           doma_stats *stats);
+
+/* Kinetics data for transient solves (MaterialProps _g_inv_v, _num_d_groups,
+ * _n_g_chi_d, _g_n_beta, _n_lambda, include/InputParameters.h:24-55): inv_v[M][G];
+ * num_delayed[M] in [0, DOMA_MAX_DELAYED]; chi_d[M][G][DOMA_MAX_DELAYED],
+ * beta[M][G][DOMA_MAX_DELAYED], lambda[M][DOMA_MAX_DELAYED] (entries past a
+ * material's count are ignored).  A material with delayed groups needs
+ * nu_sigma_f (doma_set_fission); the prompt fraction given there should be
+ * 1 - sum_d beta.  doma_set_materials clears it. */
+int doma_set_kinetics(doma_ctx *ctx, int32_t num_mats, const double *inv_v,
+                      const int32_t *num_delayed, const double *chi_d, const double *beta,
+                      const double *lambda);
+
+/* Step sources of a transient (the deck's <transients><source_step>,
+ * InputParameters.cpp:419-500), per material: type[M] (DOMA_STEP_*),
+ * insert_at[M], remove_at[M], src[M][G].  doma_set_materials clears them. */
+int doma_set_step_sources(doma_ctx *ctx, int32_t num_mats, const int32_t *type,
+                          const double *insert_at, const double *remove_at, const double *src);
+
+/* Transient from a zero initial condition (solveTransient with ics="zero",
+ * TransportSolver3D.cpp:88-260 and the 2D/1D counterparts): begin sets phi,
+ * the previous-step flux and the precursors to zero, dt = (t1 - t0) / steps and
+ * t = 0 (the reference's step clock starts at zero).  Each doma_transient_step
+ * solves one step as a fixed-source problem at time t with the implicit-Euler
+ * terms (sigma_t + inv_v/dt in the sweep; inv_v/dt phi_last, the delayed source
+ * chi_d lambda C and the step sources in the multigroup source), then advances
+ * the precursors C_d = S_d/lambda (1 - e^{-lambda dt}) + C_d e^{-lambda dt} and t.
+ * *t_solved (may be NULL) receives the step's time; the step's flux is read with
+ * doma_get_flux and the precursors with doma_get_dnps. */
+int doma_transient_begin(doma_ctx *ctx, double t0, double t1, int32_t steps);
+int doma_transient_step(doma_ctx *ctx, double fs_tol, int32_t fs_max_it, double mg_tol,
+                        int32_t mg_max_it, double *t_solved, doma_stats *stats);
+
+/* Precursor concentrations [nd_max][N] (nd_max = largest num_delayed; zero where
+ * a cell's material has fewer groups, as BrickMesh3D::dumpDNPsToTextFile). */
+int doma_get_dnps(doma_ctx *ctx, double *dnps, int32_t *nd_max);
 
 int doma_get_flux(doma_ctx *ctx, double *phi);
 
@@ -298,14 +341,16 @@ typedef struct doma_deck_info {
  * with the reference's message in doma_last_error(). */
 int doma_deck_read(const char *path, doma_deck_info *out);
 
-/* Run a fixed-source (solve_type="steady") or eigenvalue (solve_type="eigen")
- * deck end to end like the reference's main() (src/main.cpp:51-298): mesh,
+/* Run a fixed-source (solve_type="steady"), eigenvalue ("eigen") or transient
+ * ("transient") deck end to end like the reference's main() (src/main.cpp:51-298): mesh,
  * solver by dimension and scheme, GPU solve; with a non-NULL output_base the
  * reference's text files <base>_dims.txt, _blocks.txt, _meshx/y/z.txt,
  * _g<g>_flux.txt are written.  phi (may be NULL) receives [G][N] of capacity
  * phi_len doubles; *converged is the reference's solveFixedSource /
- * solveEigenvalue result and stats->k_eff the eigenvalue.  tuning may be NULL;
- * autotune != 0 runs doma_autotune first.  Transient decks are rejected. */
+ * solveEigenvalue / solveTransient result and stats->k_eff the eigenvalue.  A
+ * transient deck (ics="zero") writes <base>_t<step>_g<g>_flux.txt and
+ * _d<d>_dnps.txt per step; phi and stats are then the last step's.  tuning may be
+ * NULL; autotune != 0 runs doma_autotune first. */
 int doma_run_deck(const char *path, const char *output_base, int32_t precision,
                   const doma_tuning *tuning, int32_t autotune, double *phi, int64_t phi_len,
                   doma_stats *stats, int32_t *converged);

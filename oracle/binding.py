@@ -29,7 +29,9 @@ class OracleProblem(C.Structure):
         ("num_mats", C.c_int), ("mat", _ip), ("sigt", _dp), ("sigs", _dp), ("src", _dp),
         ("has_src", _u8p), ("dirs", C.c_int), ("mu", _dp), ("eta", _dp), ("xi", _dp),
         ("w", _dp), ("chi", _dp), ("prod", _dp), ("prompt", _dp), ("has_chi", _u8p),
-        ("has_prod", _u8p), ("dsa", C.c_int),
+        ("has_prod", _u8p), ("dsa", C.c_int), ("inv_v", _dp), ("nd", _ip), ("chi_d", _dp),
+        ("beta", _dp), ("lambda_", _dp), ("step_type", _ip), ("step_in", _dp), ("step_out", _dp),
+        ("step_src", _dp),
     ]
 
 
@@ -76,6 +78,9 @@ def lib():
         h.oracle_solve_eigenvalue.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
                                               C.c_double, C.c_int, C.c_double, C.c_int, _dp,
                                               _dp, C.POINTER(OracleStats)]
+        h.oracle_solve_transient.argtypes = [C.POINTER(OracleProblem), C.c_double, C.c_int,
+                                             C.c_double, C.c_int, C.c_double, C.c_double,
+                                             C.c_int, C.c_int, _dp, _dp, C.POINTER(OracleStats)]
         h.oracle_tridiag_solve.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         h.oracle_dsa_system.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int] + [_dp] * 9
         h.oracle_source_iteration.argtypes = [C.POINTER(OracleProblem), C.c_int, _dp, C.c_double,
@@ -104,6 +109,15 @@ class _Bound:
             self.s.has_chi = f.has_chi.ctypes.data_as(_u8p)
             self.s.has_prod = f.has_prod.ctypes.data_as(_u8p)
         self.s.dsa = int(dsa)
+        kn = p.kinetics
+        if kn is not None:
+            self.s.inv_v, self.s.nd = _d(kn.inv_v), kn.nd.ctypes.data_as(_ip)
+            self.s.chi_d, self.s.beta, self.s.lambda_ = _d(kn.chi_d), _d(kn.beta), _d(kn.lambda_)
+        st = p.steps
+        if st is not None:
+            self.s.step_type = st.type.ctypes.data_as(_ip)
+            self.s.step_in, self.s.step_out, self.s.step_src = (_d(st.insert_at),
+                                                                _d(st.remove_at), _d(st.src))
 
 
 def legendre(n):
@@ -171,6 +185,27 @@ def solve_eigenvalue(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, k_tol, nthre
              "si_residual": list(st.si_residual_last[: b.p.groups]),
              "k_eff": st.k_eff, "k_residual": st.k_residual}
     return k.value, phi.reshape(b.p.groups, b.p.N), stats
+
+
+def solve_transient(problem, fs_tol, fs_max_it, mg_tol, mg_max_it, t0, t1, steps, nthreads=1,
+                    dsa=False):
+    """Zero-IC transient: (ok, phi[steps, G, N], dnps[steps, 8, N], stats of the last step)."""
+    b = _Bound(problem, dsa)
+    if b.p.kinetics is None:
+        raise ValueError("a transient needs kinetics data")
+    G, N = b.p.groups, b.p.N
+    phi = np.zeros(steps * G * N)
+    dnp = np.zeros(steps * 8 * N)
+    st = OracleStats()
+    ok = lib().oracle_solve_transient(C.byref(b.s), fs_tol, fs_max_it, mg_tol, mg_max_it, t0, t1,
+                                      steps, nthreads, _d(phi), _d(dnp), C.byref(st))
+    if ok < 0:
+        raise RuntimeError("oracle_solve_transient failed")
+    stats = {"success": bool(st.success), "mg_iterations": st.mg_iterations,
+             "mg_residual": st.mg_residual, "sweeps": st.sweeps,
+             "si_iterations": list(st.si_iterations_last[:G]),
+             "si_residual": list(st.si_residual_last[:G])}
+    return bool(ok), phi.reshape(steps, G, N), dnp.reshape(steps, 8, N), stats
 
 
 def source_iteration(problem, g, qmg, fs_tol, fs_max_it, nthreads=1, trace=False):

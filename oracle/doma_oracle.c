@@ -550,6 +550,11 @@ typedef struct {
   double *prev;  /* [G][N] previous MG iterate */
   double *mom;   /* 1D DSA: [3][N] current, right and left interface fluxes */
   int eigen;     /* solving the eigenvalue problem (selects the DSA scheme) */
+  const oracle_problem *ps;  /* problem the sweeps see (sigma_t + inv_v/dt in a transient) */
+  int transient;
+  double t, dt;
+  double *last;  /* [G][N] previous step's flux */
+  double *dnp;   /* [8][N] precursors */
   int sweeps;
 } solver_t;
 
@@ -582,6 +587,15 @@ static void update_mg_source_k(solver_t *S, int g, int eigen, double k)
       acc = p->dims == 1 ? 0.0 : S->q[c];
     else
       acc = p->has_src[m] ? norm_src(p, p->src[(long)m * G + g]) : 0.0;
+    const double nrm = p->dims == 3 ? 0.25 : 0.5;
+    /* step source at the step's time (TransportSolver3D.cpp:353-369) */
+    if (S->transient && p->step_type && p->step_type[m] >= 0) {
+      const int ty = p->step_type[m];
+      const double a = p->step_in[m], b = p->step_out[m], t = S->t;
+      const int on = ty == 0 ? (a <= t && t <= b) : (ty == 1 ? a <= t : t <= b);
+      acc += on ? norm_src(p, p->step_src[(long)m * G + g]) : 0.0;
+    }
+    const int nd = S->transient ? p->nd[m] : 0;
     for (int gp = 0; gp < G; ++gp) {
       const double f = S->phi[(long)gp * N + c];
       if (gp != g)
@@ -592,6 +606,17 @@ static void update_mg_source_k(solver_t *S, int g, int eigen, double k)
           acc += fission_term(p, 1.0, chi, pr, f) / k;
         else
           acc += fission_term(p, p->prompt[(long)m * G + gp], chi, pr, f);
+      }
+      /* delayed and time-derivative sources, inside the g' loop as the reference
+       * has them (TransportSolver3D.cpp:388-396) */
+      for (int d = 0; d < nd; ++d) {
+        const double v = nrm * p->chi_d[((long)m * G + g) * ORACLE_MAX_DELAYED + d] *
+                         S->dnp[(long)d * N + c] * p->lambda[(long)m * ORACLE_MAX_DELAYED + d];
+        acc += p->dims == 1 ? v : v / M_PI;
+      }
+      if (S->transient) {
+        const double v = nrm * S->last[(long)g * N + c] * p->inv_v[(long)m * G + g] / S->dt;
+        acc += p->dims == 1 ? v : v / M_PI;
       }
     }
     S->dphi[c] = 0.0;
@@ -784,7 +809,7 @@ static int source_iteration(solver_t *S, int g, double tol, int max_it, int *its
   double r = 0.0;
   do {
     const int dsa = p->dims == 1 && p->dsa;
-    sweep_with(p, g, S->q, S->dphi, S->T, &S->s, dsa ? S->mom : NULL);
+    sweep_with(S->ps, g, S->q, S->dphi, S->T, &S->s, dsa ? S->mom : NULL);
     if (dsa) synthetic_acceleration(S, g);
     S->sweeps++;
     r = scattering_residual(S, g);
@@ -821,6 +846,7 @@ static int solver_init(solver_t *S, const oracle_problem *p, int nthreads)
   const long N = ncells(p);
   memset(S, 0, sizeof(*S));
   S->p = p;
+  S->ps = p;
   S->T = team(nthreads);
   S->q = calloc(N, sizeof(double));
   S->dphi = calloc(N, sizeof(double));
@@ -954,6 +980,87 @@ int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_i
   }
   solver_free(&S);
   return success;
+}
+
+/* stepDNPs (TransportSolver3D.cpp:230-260). */
+static void step_dnps(solver_t *S)
+{
+  const oracle_problem *p = S->p;
+  const long N = ncells(p);
+  const int G = p->groups, K = ORACLE_MAX_DELAYED;
+  for (long c = 0; c < N; ++c) {
+    const int m = p->mat[c];
+    for (int d = 0; d < p->nd[m]; ++d) {
+      double src = 0.0;
+      for (int g = 0; g < G; ++g)
+        src += p->prod[(long)m * G + g] * S->phi[(long)g * N + c] *
+               p->beta[((long)m * G + g) * K + d];
+      const double lam = p->lambda[(long)m * K + d];
+      const double e = exp(-lam * S->dt);
+      S->dnp[(long)d * N + c] = (src / lam) * (1.0 - e) + S->dnp[(long)d * N + c] * e;
+    }
+  }
+}
+
+int oracle_solve_transient(const oracle_problem *p, double fs_tol, int fs_max_it, double mg_tol,
+                           int mg_max_it, double t0, double t1, int steps, int nthreads,
+                           double *phi_steps, double *dnp_steps, oracle_stats *stats)
+{
+  if (!p->inv_v || !p->nd || steps < 1) return -1;
+  solver_t S;
+  if (solver_init(&S, p, nthreads)) { solver_free(&S); return -1; }
+  const long N = ncells(p);
+  const int G = p->groups, K = ORACLE_MAX_DELAYED;
+  /* the sweeps see sigma_t + inv_v / dt (DiamondDifference3D.h:25-26) */
+  S.dt = (t1 - t0) / (double)steps;
+  double *sig = malloc(sizeof(double) * (size_t)p->num_mats * G);
+  for (long e = 0; e < (long)p->num_mats * G; ++e) sig[e] = p->sigt[e] + p->inv_v[e] / S.dt;
+  oracle_problem ps = *p;
+  ps.sigt = sig;
+  S.ps = &ps;
+  S.transient = 1;
+  S.last = calloc((size_t)G * N, sizeof(double));
+  S.dnp = calloc((size_t)K * N, sizeof(double));
+  S.t = 0.0;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  int all_ok = 1;
+  for (int step = 0; step < steps; ++step) {
+    int mg_it = 0, ok = 1;
+    double r = 0.0;
+    do {
+      for (int g = 0; g < G; ++g) {
+        update_mg_source(&S, g);
+        int its = 0;
+        double res = 0.0;
+        int conv = source_iteration(&S, g, fs_tol, fs_max_it, &its, &res, NULL);
+        if (stats && g < 64) {
+          stats->si_iterations_last[g] = its;
+          stats->si_residual_last[g] = res;
+        }
+        if (!conv) { ok = 0; break; }
+      }
+      if (!ok) break;
+      r = mg_residual(&S);
+      mg_it++;
+    } while (mg_it < mg_max_it && mg_tol < r && G > 1);
+    if (stats) {
+      stats->success = ok && mg_it < mg_max_it;
+      stats->mg_iterations = mg_it;
+      stats->mg_residual = r;
+      stats->sweeps = S.sweeps;
+    }
+    if (!(ok && mg_it < mg_max_it)) { all_ok = 0; break; }
+    step_dnps(&S);
+    if (phi_steps) memcpy(phi_steps + (size_t)step * G * N, S.phi, sizeof(double) * (size_t)G * N);
+    if (dnp_steps) memcpy(dnp_steps + (size_t)step * K * N, S.dnp, sizeof(double) * (size_t)K * N);
+    /* updateStepFluxes (TransportSolver3D.cpp:204-228) */
+    memcpy(S.last, S.phi, sizeof(double) * (size_t)G * N);
+    memset(S.phi, 0, sizeof(double) * (size_t)G * N);
+    S.t += S.dt;
+  }
+  free(sig); free(S.last); free(S.dnp);
+  solver_free(&S);
+  return all_ok;
 }
 
 int oracle_source_iteration(const oracle_problem *p, int g, const double *qmg, double fs_tol,

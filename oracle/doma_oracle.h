@@ -68,7 +68,22 @@ typedef struct oracle_problem {
   const uint8_t *has_chi;
   const uint8_t *has_prod;
   int dsa;                  /* 1D: diffusion synthetic acceleration after every sweep */
+  /* Transient data (oracle_solve_transient only; ORACLE_MAX_DELAYED slots per
+   * material and group): inv_v [M][G]; nd [M]; chi_d, beta [M][G][8]; lambda
+   * [M][8]; step_type [M] (-1 none, 0 both, 1 insert, 2 remove), step_in,
+   * step_out [M], step_src [M][G] (NULL step_type: no step sources). */
+  const double *inv_v;
+  const int32_t *nd;
+  const double *chi_d;
+  const double *beta;
+  const double *lambda;
+  const int32_t *step_type;
+  const double *step_in;
+  const double *step_out;
+  const double *step_src;
 } oracle_problem;
+
+#define ORACLE_MAX_DELAYED 8
 
 typedef struct oracle_stats {
   int success;              /* reference solveFixedSource return value */
@@ -127,6 +142,16 @@ int oracle_solve_fixed_source(const oracle_problem *p, double fs_tol, int fs_max
 int oracle_solve_eigenvalue(const oracle_problem *p, double fs_tol, int fs_max_it,
                             double mg_tol, int mg_max_it, double k_tol, int nthreads,
                             double *phi_out, double *k_eff, oracle_stats *stats);
+
+/* Transient from a zero initial condition (TransportSolver3D.cpp:88-260, ics =
+ * zero; 2D/1D counterparts): dt = (t1 - t0) / steps, t from 0; per step the
+ * multigroup solve with the time terms, then the precursor update.  phi_steps
+ * [steps][G][N] and dnp_steps [steps][8][N] (may be NULL) receive every step's
+ * flux and precursors; stats describe the last step solved.  Returns 1 if every
+ * step converged. */
+int oracle_solve_transient(const oracle_problem *p, double fs_tol, int fs_max_it, double mg_tol,
+                           int mg_max_it, double t0, double t1, int steps, int nthreads,
+                           double *phi_steps, double *dnp_steps, oracle_stats *stats);
 
 /* Tridiagonal solve by Gaussian elimination with partial pivoting (row
  * interchanges between neighbours, as LAPACK dgtsv).  lo[i] = A[i][i-1],

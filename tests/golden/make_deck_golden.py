@@ -65,6 +65,8 @@ DECKS = {
     "1D_NPRE_555_mp3_3g_src_dsa": "cases/1D_NPRE_555_mp3_3g_src_dsa.xml",
     "1D_NPRE_555_mp3_3g_eig_dsa": "cases/1D_NPRE_555_mp3_3g_eig_dsa.xml",
     "1D_NPRE_555_mp1": "cases/1D_NPRE_555_mp1.xml",
+    "2D_assembly_tr_short": "cases/2D_assembly_transient/2D_assembly_tr.xml",
+    "2D_assembly_tr_remove": "cases/2D_assembly_transient/2D_assembly_tr.xml",
     "2D_assembly_eig": "cases/2D_assembly_eig.xml",
     "2D_assembly_eig_coarse": "cases/2D_assembly_eig.xml",
     "3D_assembly_fs_coarse": "cases/3D_assembly_fs.xml",
@@ -74,6 +76,13 @@ DECKS = {
 _ASSEMBLY_XY = "8, " + "2, " * 11 + "8"
 COARSE = {
     "2D_assembly_eig_coarse": {"refinement": ("mesh", "0")},
+    # the transient deck's first 5 of 1000 steps (same dt = 0.01) on the unrefined mesh
+    "2D_assembly_tr_short": {"refinement": ("mesh", "0"), "solver": ("t1", "0.05"),
+                             "solver:t_steps": ("t_steps", "5")},
+    # 4 steps with the step source removed after the third (t = 0, .01, .02 on)
+    "2D_assembly_tr_remove": {"refinement": ("mesh", "0"), "solver": ("t1", "0.04"),
+                              "solver:t_steps": ("t_steps", "4"),
+                              "source_step": ("remove_at", "0.02")},
     "3D_assembly_fs_coarse": {"ix": ("vals", _ASSEMBLY_XY), "iy": ("vals", _ASSEMBLY_XY),
                               "iz": ("vals", "4, 1, 4")},
     "3D_assembly_eig_coarse": {"ix": ("vals", _ASSEMBLY_XY), "iy": ("vals", _ASSEMBLY_XY),
@@ -87,6 +96,8 @@ def write_spec(p, path, run):
     lines = [f"{p['dims']} {p['G']} {p['scheme']} {p['np']} {p['na']} {p['fs_max_it']} "
              f"{p['mg_max_it']} {THREADS} {run}",
              f"{p['fs_tol']!r} {p['mg_tol']!r} {p['k_tol']!r}"]
+    if run == 3:
+        lines.append(f"{p['t0']!r} {p['t1']!r} {p['t_steps']}")
     for ax in "xyz"[:p["dims"]]:
         d, n = p["d" + ax], p["i" + ax]
         lines.append(f"{len(d)} " + " ".join(repr(v) for v in d) + " " + " ".join(map(str, n)))
@@ -94,10 +105,16 @@ def write_spec(p, path, run):
     lines.append(str(len(p["materials"])))
     for bid, d in sorted(p["materials"].items()):
         flags = [d["src"] is not None, d["chi"] is not None, d["prod"] is not None]
+        nd = d["nd"]
         vals = d["sigt"] + d["sigs"] + (d["src"] or []) + (d["chi"] or []) + \
-            (d["prod"] or []) + (d["beta"] if d["nd"] else [])
-        lines.append(f"{bid} " + " ".join(str(int(f)) for f in flags) + f" {d['nd']} " +
-                     " ".join(repr(v) for v in vals))
+            (d["prod"] or []) + (d["beta"] + d["chi_d"] + d["lambda"] if nd else []) + \
+            (d["inv_v"] or [])
+        lines.append(f"{bid} " + " ".join(str(int(f)) for f in flags) +
+                     f" {nd} {int(d['inv_v'] is not None)} " + " ".join(repr(v) for v in vals))
+    lines.append(str(len(p["steps"])))
+    for bid, st in sorted(p["steps"].items()):
+        lines.append(f"{bid} {st['type']} {st['insert']!r} {st['remove']!r} " +
+                     " ".join(repr(v) for v in st["src"]))
     Path(path).write_text("\n".join(lines) + "\n")
 
 
@@ -110,10 +127,18 @@ def _files(base, dims, G):
 def _coarsen(name, src, tmp):
     """The reference deck with the COARSE mesh attributes replaced."""
     tree = ET.parse(src)
-    mesh = tree.getroot().find("mesh")
+    root = tree.getroot()
+    mesh = root.find("mesh")
     for tag, (attr, val) in COARSE[name].items():
-        el = mesh if tag == "refinement" else mesh.find(tag)
-        el.set("refinement" if tag == "refinement" else attr, val)
+        tag = tag.split(":")[0]
+        if tag == "refinement":
+            mesh.set("refinement", val)
+        elif tag == "solver":
+            root.find("solver").set(attr, val)
+        elif tag == "source_step":
+            root.find("transients").find("source_step").set(attr, val)
+        else:
+            mesh.find(tag).set(attr, val)
     out = Path(tmp) / f"{name}.src.xml"
     tree.write(out, encoding="unicode")
     return out
@@ -124,17 +149,28 @@ def make(name, tmp):
     if name in COARSE:
         src = _coarsen(name, src, tmp)
     root, p = read_deck(src)
-    assert p["mode"] in ("steady", "eigen"), name
     OUT.mkdir(exist_ok=True)
     ET.ElementTree(root).write(OUT / f"{name}.xml", encoding="unicode")
-    run = 0 if p["dims"] == 1 else (2 if p["mode"] == "eigen" else 1)
+    run = 0 if p["dims"] == 1 else {"steady": 1, "eigen": 2, "transient": 3}[p["mode"]]
     spec, base = Path(tmp) / f"{name}.spec", str(Path(tmp) / name)
     write_spec(p, spec, run)
     subprocess.run([str(HARNESS), str(spec), base], check=True)
     raw = Path(base + ".bin").read_bytes()
     ok = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
-    phi = np.frombuffer(raw[4:], dtype=np.float64)
+    nphi = p["G"] * int(np.prod([sum(p["i" + ax]) for ax in "xyz"[:p["dims"]]]))
+    phi = np.frombuffer(raw[4:4 + 8 * nphi], dtype=np.float64)
     files = _files(base, p["dims"], p["G"])
+    if run == 3:
+        # the transient's initial dump holds the zero initial condition; its
+        # per-step outputs are <base>_t<step>_g<g>_flux.txt and _d<d>_dnps.txt
+        rest = raw[4 + 8 * nphi:]
+        nd = int(np.frombuffer(rest[:4], dtype=np.int32)[0])
+        dnps = np.frombuffer(rest[4:], dtype=np.float64).reshape(nd, -1)
+        for step in range(p["t_steps"]):
+            for g in range(p["G"]):
+                files[f"t{step}_g{g}_flux"] = Path(f"{base}_t{step}_g{g}_flux.txt")
+            for d in range(nd):
+                files[f"t{step}_d{d}_dnps"] = Path(f"{base}_t{step}_d{d}_dnps.txt")
     if not run:
         files = {k: v for k, v in files.items() if not k.endswith("_flux")}
     arrays = dict(ref_success=ok, dims=p["dims"], groups=p["G"])
@@ -149,9 +185,17 @@ def make(name, tmp):
         phi = phi.reshape(p["G"], N)
         # The reference writes each value with operator<< at precision 6, which
         # is printf's %.6g: check that equivalence once here, on its own files.
+        last = f"t{p['t_steps'] - 1}_" if run == 3 else ""
         for g in range(p["G"]):
-            text = files[f"g{g}_flux"].read_text().splitlines()
+            text = files[f"{last}g{g}_flux"].read_text().splitlines()
             assert text == ["%.6g" % v for v in phi[g]], (name, g)
+        if run == 3:
+            arrays["dnps"] = dnps
+            arrays["t_steps"] = p["t_steps"]
+            # the reference's per-step text files, for a line-by-line comparison
+            for k, f in files.items():
+                if k.startswith("t"):
+                    arrays[f"text_{k}"] = np.array(f.read_text().splitlines())
         if phi.size <= FULL_LIMIT:
             arrays["phi"] = phi
         else:

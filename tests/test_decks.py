@@ -30,7 +30,7 @@ sys.path.insert(0, str(GOLDEN))
 from deck_xml import fission_arrays, read_deck  # noqa: E402  (xml.etree reader, independent of the product)
 
 DECK_NAMES = sorted(p.stem for p in DECKS.glob("*.xml"))
-SOLVED = [n for n in DECK_NAMES if not n.startswith("1D")]
+SOLVED = [n for n in DECK_NAMES if not n.startswith("1D") and "_tr_" not in n]
 MESH_ONLY = [n for n in DECK_NAMES if n.startswith("1D")]
 
 
