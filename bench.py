@@ -255,6 +255,36 @@ def precision_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def deck_mode(args):
+    """One input deck end to end on the GPU, as the reference's main() runs it
+    (fixed-source, eigenvalue or transient; no text outputs): wall time, sweeps,
+    the end-to-end grind time per cell-direction-group sweep, k for eigenvalue
+    decks, the step count for transients.  --autotune picks the kernel shape
+    first (not timed)."""
+    from doma_amd import capi
+
+    deck = Path(args.deck)
+    info = capi.deck_read(deck)
+    N = info["nx"] * info["ny"] * info["nz"]
+    log(f"bench: deck {deck.name} mode {info['mode']} N={N} groups={info['groups']}")
+    if args.autotune:
+        capi.run_deck(deck, None, autotune=True)  # warm-up + autotune (cached per shape)
+    t0 = time.perf_counter()
+    conv, phi, st = capi.run_deck(deck, None, autotune=args.autotune)
+    wall = time.perf_counter() - t0
+    dims = info["dims"]
+    dirs = {3: 8, 2: 4, 1: 2}[dims] * (info["np"] * info["na"] if dims > 1 else info["np"])
+    out = {"deck": deck.name, "mode": ["fixed_source", "eigenvalue", "transient"][info["mode"]],
+           "cells": N, "groups": info["groups"], "directions": dirs, "converged": conv,
+           "wall_s": round(wall, 3), "sweeps_last_solve": st["sweeps"],
+           "mg_iterations": st["mg_iterations"], "si_iterations_last": st["si_iterations"],
+           "sweep_ms_per_sweep": round(st["sweep_ms"] / max(st["sweeps"], 1), 4)}
+    if info["mode"] == 1:
+        out["k_eff"] = st["k_eff"]
+        out["grind_ns_end_to_end"] = round(wall * 1e9 / (st["sweeps"] * N * dirs), 5)
+    print(json.dumps(out), flush=True)
+
+
 def learned_mode(args):
     print(json.dumps(learned_compare(args.workload if args.workload in ("diffusive", "high_scatter")
                                      else "high_scatter", args.n, args.groups, args.dd, args.every,
@@ -285,8 +315,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain solve")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
-    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned", "precision"],
-                    default="solve")
+    ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned", "precision",
+                                       "deck"], default="solve")
+    ap.add_argument("--deck", default=None, help="input deck (deck mode)")
+    ap.add_argument("--autotune", action="store_true", help="deck mode: autotune the sweep")
     ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
     ap.add_argument("--no-reference", action="store_true")
     ap.add_argument("--dd", action="store_true", help="diamond difference (learned mode)")
@@ -316,6 +348,8 @@ def main():
         return learned_mode(args)
     if args.mode == "precision":
         return precision_mode(args)
+    if args.mode == "deck":
+        return deck_mode(args)
 
     from doma_amd import workloads
     from doma_amd.capi import FP64, Context

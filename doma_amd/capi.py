@@ -106,6 +106,7 @@ SIGNATURES = [
     ("doma_set_materials", C.c_int, [C.c_void_p, _ip, C.c_int32, _dp, _dp, _dp, _u8p]),
     ("doma_set_fission", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _u8p, _u8p]),
     ("doma_set_dsa", C.c_int, [C.c_void_p, C.c_int32]),
+    ("doma_set_si_mode", C.c_int, [C.c_void_p, C.c_int32]),
     ("doma_set_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("doma_get_tuning", C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     ("doma_sweep", C.c_int, [C.c_void_p, C.c_int32, _dp, _dp]),
@@ -464,6 +465,10 @@ class Context:
         phi = np.zeros(self.problem.groups * self.problem.N)
         _check(lib().doma_get_flux(self.h, _d(phi)), "doma_get_flux")
         return phi.reshape(self.problem.groups, self.problem.N)
+
+    def set_si_mode(self, device_loop: bool = True):
+        """Stopping rule of source iteration on the device (default) or per sweep on the host."""
+        _check(lib().doma_set_si_mode(self.h, int(device_loop)), "doma_set_si_mode")
 
     def set_dsa(self, enable: bool = True):
         """1D diffusion synthetic acceleration after every sweep."""

@@ -418,9 +418,11 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.q = q;
     a.sig = c->d_sig;
     a.part = c->d_part;
+    a.skip = c->si_device ? c->d_si_state : nullptr;
     return launch_sweep1d(a, c->precision, c->stream);
   }
   BrickArgs a = brick_args(c, q);
+  a.skip = c->si_device ? c->d_si_state : nullptr;
   if (c->schedule == 1)
   {
     a.ntasks = c->ntasks;
@@ -481,6 +483,106 @@ check_ctx(const doma_ctx * c, bool need_materials)
   return DOMA_OK;
 }
 
+// 1D DSA after a sweep: the accelerated flux replaces the swept one in af.
+int
+enqueue_dsa(doma_ctx * c, IterArgs & af)
+{
+  if (!(c->dsa && c->dims == 1))
+    return DOMA_OK;
+  // syntheticAcceleration follows the sweep (TransportSolver1D.cpp:513-514).
+  DsaArgs d{};
+  d.dirs = c->dirs;
+  d.dirtab = c->d_dirtab;
+  d.beta_left = c->dsa_beta[0];
+  d.beta_right = c->dsa_beta[1];
+  d.work = c->d_dsa + c->N;
+  d.out = c->d_dsa;
+  af.eigen = c->solving_eigen;
+  int rc = launch_dsa1d(af, d, c->stream);
+  af.dsa = c->d_dsa;
+  return rc;
+}
+
+// Source iteration with the stopping rule evaluated on the device: the host
+// enqueues a chunk of iterations (sweep, DSA, finish, residual) and waits once
+// per chunk.  The residual kernel raises a flag after the iteration that meets
+// !(tol < r) or it == max_it, and every later kernel of the chunk returns at
+// entry, so iterates, iteration count and last residual equal those of one host
+// round trip per iteration.  Chunks grow 1, 2, 4, 8 so short iterations enqueue
+// few idle launches.
+int
+si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, int * conv,
+               double * sweep_ms, int * sweeps, const double * first_corr)
+{
+  constexpr int kMaxChunk = 8;
+  int rc = DOMA_OK;
+  if (c->si_events.empty())
+  {
+    c->si_events.resize(2 * kMaxChunk);
+    for (auto & e : c->si_events)
+      DOMA_HIP(hipEventCreate(&e));
+  }
+  IterArgs a = iter_args(c, g);
+  a.si_state = c->d_si_state;
+  a.tol = tol;
+  a.max_it = max_it;
+  if ((rc = launch_fill_i32(c->d_si_state, 4, 0, c->stream)))
+    return rc;
+  c->si_device = true;
+  struct Off  // sweeps launched after the loop run unconditionally again
+  {
+    doma_ctx * c;
+    ~Off() { c->si_device = false; }
+  } off{c};
+  int done_its = 0, chunk = 1;
+  bool done = false;
+  while (!done)
+  {
+    const int n = std::min(chunk, max_it - done_its);
+    for (int k = 0; k < n; ++k)
+    {
+      IterArgs af = a;
+      if (done_its + k == 0 && first_corr)
+        af.corr = first_corr;  // warm start: the first increment is measured from prev_g
+      DOMA_HIP(hipEventRecord(c->si_events[2 * k], c->stream));
+      if ((rc = run_sweep(c, g)))
+        return rc;
+      DOMA_HIP(hipEventRecord(c->si_events[2 * k + 1], c->stream));
+      if ((rc = enqueue_dsa(c, af)) || (rc = launch_finish(af, c->red_blocks, c->stream)) ||
+          (rc = launch_residual_final(af, c->red_blocks, c->stream)))
+        return rc;
+    }
+    DOMA_HIP(hipMemcpyAsync(c->h_si_state, c->d_si_state, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    DOMA_HIP(hipStreamSynchronize(c->stream));
+    DOMA_HIP(hipGetLastError());
+    const int ran = c->h_si_state[1] - done_its;
+    for (int k = 0; k < ran && k < n; ++k)
+    {
+      float ms = 0.f;
+      DOMA_HIP(hipEventElapsedTime(&ms, c->si_events[2 * k], c->si_events[2 * k + 1]));
+      if (sweep_ms)
+        *sweep_ms += ms;
+    }
+    if (sweeps)
+      *sweeps += ran;
+    done_its = c->h_si_state[1];
+    if (c->h_res[3] != 0.0)
+      return reset_after_sweep_error(c);
+    done = c->h_si_state[0] != 0 || done_its >= max_it;
+    chunk = std::min(2 * chunk, kMaxChunk);
+  }
+  if (its)
+    *its = done_its;
+  if (res)
+    *res = c->h_res[0];
+  if (conv)
+    *conv = done_its < max_it ? 1 : 0;
+  return DOMA_OK;
+}
+
 // Scattering source iteration of group g from the state prepared by the caller
 // (TransportSolver3D.cpp:458-497): sweep, residual, source update, until
 // it == max_it or the residual is <= tol.
@@ -494,6 +596,8 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
   // the corrected state; if that sweep's residual is not below the one before
   // the correction, the correction is undone (the sweep still counts).
   const bool learned = c->net.ready && c->net.every > 0 && c->dims == 3;
+  if (!learned && c->si_chunked)
+    return si_loop_device(c, g, tol, max_it, its, res, conv, sweep_ms, sweeps, first_corr);
   IterArgs a = iter_args(c, g);
   if (learned)
     a.dcur = c->d_dcur;
@@ -516,21 +620,8 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     if (rc)
       return rc;
     DOMA_HIP(hipEventRecord(c->ev1, c->stream));
-    if (c->dsa && c->dims == 1)
-    {
-      // syntheticAcceleration follows the sweep (TransportSolver1D.cpp:513-514).
-      DsaArgs d{};
-      d.dirs = c->dirs;
-      d.dirtab = c->d_dirtab;
-      d.beta_left = c->dsa_beta[0];
-      d.beta_right = c->dsa_beta[1];
-      d.work = c->d_dsa + c->N;
-      d.out = c->d_dsa;
-      af.eigen = c->solving_eigen;
-      if ((rc = launch_dsa1d(af, d, c->stream)))
-        return rc;
-      af.dsa = c->d_dsa;
-    }
+    if ((rc = enqueue_dsa(c, af)))
+      return rc;
     if ((rc = launch_finish(af, c->red_blocks, c->stream)) ||
         (rc = launch_residual_final(af, c->red_blocks, c->stream)))
       return rc;
@@ -758,6 +849,9 @@ create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
       (rc = dev_alloc(&c->d_prev, N * c->G)))
     return cleanup(rc);
   c->red_blocks = reduction_blocks(c->N);
+  if ((rc = dev_alloc(&c->d_si_state, 4)) ||
+      hipHostMalloc(reinterpret_cast<void **>(&c->h_si_state), 4 * sizeof(int)) != hipSuccess)
+    return cleanup(rc ? rc : fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
   if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 8)) ||
       (rc = dev_alloc(&c->d_err, 1)) || (rc = dev_alloc(&c->d_ticket, 1)))
     return cleanup(rc);
@@ -859,6 +953,11 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_keep);
   if (c->h_res)
     (void)hipHostFree(c->h_res);
+  if (c->h_si_state)
+    (void)hipHostFree(c->h_si_state);
+  dev_free(c->d_si_state);
+  for (auto e : c->si_events)
+    (void)hipEventDestroy(e);
   if (c->ev0)
     (void)hipEventDestroy(c->ev0);
   if (c->ev1)
@@ -972,6 +1071,16 @@ doma_set_fission(doma_ctx * c, int32_t M, const double * chi, const double * pro
   DOMA_HIP(hipMemcpy(c->d_has_chi, has_chi, (size_t)M, hipMemcpyHostToDevice));
   DOMA_HIP(hipMemcpy(c->d_has_prod, has_prod, (size_t)M, hipMemcpyHostToDevice));
   c->fission = true;
+  return DOMA_OK;
+}
+
+extern "C" int
+doma_set_si_mode(doma_ctx * c, int32_t device_loop)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  c->si_chunked = device_loop != 0;
   return DOMA_OK;
 }
 

@@ -62,6 +62,11 @@ struct BrickArgs
   unsigned long long ticket_base;   // counter value at the start of this sweep
   int * error;                      // set when a bounded wait expires
   long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
+
+  // Device-side source iteration: when *skip != 0 (the iteration has converged)
+  // the launch does nothing (null: always run).  A skipped dataflow launch still
+  // advances the ticket counter by ntasks + grid, as a full one does.
+  const int * skip;
 };
 
 // Arguments of the batched persistent sweep: many 3D problems swept with one
@@ -92,6 +97,7 @@ struct Sweep1DArgs
   const double * q;
   const double * sig;
   double * part;          // [2*dirs][nx]
+  const int * skip;       // as BrickArgs::skip
 };
 
 // Kernel shapes of the brick sweep (table in sweep_impl.h).
@@ -168,6 +174,14 @@ struct IterArgs
   double * dnp;            // [DOMA_MAX_DELAYED][N] precursor concentrations
   double t;                // time of the step
   double dt;
+  // Device-side source iteration (null si_state: one host round trip per
+  // iteration).  si_state[0] = converged-or-done flag, si_state[1] = iterations
+  // done; the residual kernel sets the flag once !(tol < r), the count reaches
+  // max_it or a sweep reported an expired wait, and every later kernel of the
+  // iteration returns at entry.
+  int * si_state;
+  double tol;
+  int max_it;
 };
 
 // 1D diffusion synthetic acceleration of one sweep (see iterate.hip).
@@ -196,6 +210,7 @@ int launch_dsa1d(const IterArgs & a, const DsaArgs & d, hipStream_t s);
 // stepDNPs (TransportSolver3D.cpp:230-260): precursor update after a step.
 int launch_step_dnps(const IterArgs & a, hipStream_t s);
 int launch_fill(double * p, size_t n, double v, hipStream_t s);
+int launch_fill_i32(int * p, size_t n, int v, hipStream_t s);
 // Batched finish/residual over problem slots (see iterate.hip), training-data
 // snapshots and targets.
 int launch_batch_finish(const IterArgs * d_args, const int4 * d_map, int nblocks, hipStream_t s);
@@ -312,6 +327,12 @@ struct doma_ctx
   uint32_t * d_flags = nullptr;
   unsigned long long * d_ticket = nullptr;
   int * d_err = nullptr;
+  // Device-side source iteration: [0] done flag, [1] iterations (see IterArgs).
+  int * d_si_state = nullptr;
+  int * h_si_state = nullptr;  // pinned copy
+  bool si_device = false;      // sweeps launched now check d_si_state
+  bool si_chunked = true;      // source iteration with the stopping rule on the device
+  std::vector<hipEvent_t> si_events;
   uint32_t epoch = 0;
   unsigned long long ticket_base = 0;
   int wall_clock_khz = 100000;

@@ -203,6 +203,8 @@ __device__ __forceinline__ void
 finish_body(const IterArgs & a, int block, int nblocks)
 {
 #pragma clang fp contract(off)
+  if (a.si_state && a.si_state[0])
+    return;
   double diff = 0.0, total = 0.0;
   const int stride = nblocks * RED_THREADS;
   for (int c = block * RED_THREADS + threadIdx.x; c < a.N; c += stride)
@@ -260,6 +262,8 @@ finish_kernel(IterArgs a)
 __device__ __forceinline__ void
 residual_body(const IterArgs & a, int nblocks)
 {
+  if (a.si_state && a.si_state[0])
+    return;
   double diff = 0.0, total = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += RED_THREADS)
   {
@@ -269,10 +273,18 @@ residual_body(const IterArgs & a, int nblocks)
   block_sum2(diff, total);
   if (threadIdx.x == 0)
   {
-    a.res[0] = total > 1e-8 ? sqrt(diff) / sqrt(total) : 0.0;
+    const double r = total > 1e-8 ? sqrt(diff) / sqrt(total) : 0.0;
+    a.res[0] = r;
     a.res[1] = diff;
     a.res[2] = total;
     a.res[3] = a.err ? static_cast<double>(*a.err) : 0.0;
+    if (a.si_state)
+    {
+      // The loop condition of sourceIteration (TransportSolver3D.cpp:481).
+      const int it = ++a.si_state[1];
+      if (!(a.tol < r) || it >= a.max_it || a.res[3] != 0.0)
+        a.si_state[0] = 1;
+    }
   }
 }
 
@@ -480,6 +492,8 @@ __global__ __launch_bounds__(RED_THREADS) void
 dsa1d_kernel(IterArgs a, DsaArgs d)
 {
 #pragma clang fp contract(off)
+  if (a.si_state && a.si_state[0])
+    return;
   const int N = a.N, n = N + 1, G = a.G, g = a.g, D = d.dirs;
   double * phi = d.work;
   double * cur = phi + N;
@@ -634,6 +648,14 @@ dsa1d_kernel(IterArgs a, DsaArgs d)
   __syncthreads();
   for (int c = threadIdx.x; c < N; c += RED_THREADS)
     d.out[c] = 0.5 * (x[c] + x[c + 1]);
+}
+
+__global__ void
+fill_i32_kernel(int * p, size_t n, int v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
 }
 
 __global__ void
@@ -831,6 +853,15 @@ int
 launch_warm_start(const IterArgs & a, hipStream_t s)
 {
   hipLaunchKernelGGL(warm_start_kernel, dim3(blocks_for(a.N, 256)), dim3(256), 0, s, a);
+  return DOMA_OK;
+}
+
+int
+launch_fill_i32(int * p, size_t n, int v, hipStream_t s)
+{
+  if (n == 0)
+    return DOMA_OK;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
   return DOMA_OK;
 }
 

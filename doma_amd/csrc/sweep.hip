@@ -74,7 +74,7 @@ __global__ __launch_bounds__(1024) void
 sweep1d_kernel(Sweep1DArgs a)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * a.dirs)
+  if (t >= 2 * a.dirs || (a.skip && *a.skip))
     return;
   const int half = t / a.dirs;
   const double * d = a.dirtab + 4 * t;

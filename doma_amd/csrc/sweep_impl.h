@@ -465,6 +465,8 @@ template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
 __global__ __launch_bounds__(kShapes[SHAPE].NT) void
 brick_sweep_kernel(BrickArgs a)
 {
+  if (a.skip && *a.skip)
+    return;
   constexpr Shape S = kShapes[SHAPE];
   extern __shared__ __align__(16) unsigned char smem[];
   load_plane_table<R, DIM, S.B, S.DL, S.NDL, S.KO, S.NT>(a, smem);
@@ -515,6 +517,13 @@ template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
 __global__ __launch_bounds__(kShapes[SHAPE].NT) void
 brick_flow_kernel(BrickArgs a)
 {
+  if (a.skip && *a.skip)
+  {
+    // Keep the ticket counter where a full launch would leave it.
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      atomicAdd(a.ticket, (unsigned long long)a.ntasks + gridDim.x);
+    return;
+  }
   constexpr Shape S = kShapes[SHAPE];
   using L = Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>;
   extern __shared__ __align__(16) unsigned char smem[];

@@ -141,6 +141,14 @@ int doma_set_fission(doma_ctx *ctx, int32_t num_mats, const double *chi_p, const
  * InputParameters.cpp:403-414).  enable = 0 turns it off. */
 int doma_set_dsa(doma_ctx *ctx, int32_t enable);
 
+/* Where source iteration tests its stopping rule (no reference counterpart):
+ * device_loop = 1 (default) evaluates !(tol < r) and the iteration limit on the
+ * device and lets the host enqueue several iterations per round trip (kernels
+ * after the last iteration return at entry); 0 waits for the residual after
+ * every sweep, as the reference's loop does.  Both give the same iterates and
+ * counts.  The learned correction always uses the per-iteration loop. */
+int doma_set_si_mode(doma_ctx *ctx, int32_t device_loop);
+
 int doma_set_tuning(doma_ctx *ctx, const doma_tuning *t);
 int doma_get_tuning(const doma_ctx *ctx, doma_tuning *t);
 

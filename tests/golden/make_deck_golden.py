@@ -222,7 +222,27 @@ def make(name, tmp):
     return meta
 
 
+# Decks too large for the reference to solve here, re-serialised as benchmark
+# inputs only (tests/golden/bench_decks/<name>.xml, no fixture).
+BENCH_DECKS = {
+    "3D_assembly_eig": "cases/3D_assembly_eig.xml",
+    "3D_assembly_fs": "cases/3D_assembly_fs.xml",
+    "2D_assembly_tr": "cases/2D_assembly_transient/2D_assembly_tr.xml",
+}
+
+
+def write_bench_decks():
+    out = OUT.parent / "bench_decks"
+    out.mkdir(exist_ok=True)
+    for name, rel in BENCH_DECKS.items():
+        root, _ = read_deck(REF / rel)
+        ET.ElementTree(root).write(out / f"{name}.xml", encoding="unicode")
+
+
 def main(argv):
+    if argv == ["--bench-decks"]:
+        write_bench_decks()
+        return
     names = argv or list(DECKS)
     OUT.mkdir(exist_ok=True)
     man_path = OUT / "manifest.json"
