@@ -247,6 +247,30 @@ configure_sweep(doma_ctx * c)
     c->launches_per_sweep = 1;
     return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
   }
+  // 2D: one lane per row (schedule 2) when asked for, or by default when no
+  // brick shape or schedule is requested and the mesh fits one work-group.
+  const bool untuned = !c->tuning.shape && !c->tuning.schedule && !c->tuning.brick &&
+                       !c->tuning.threads && !c->tuning.octants && !c->tuning.dirs_per_lane;
+  if (c->dims == 2 && (c->tuning.schedule == 2 || (untuned && rows2d_fits(c->nx, c->ny))))
+  {
+    if (!rows2d_fits(c->nx, c->ny))
+      return fail(DOMA_ERR_INVALID, "the row-pipelined 2D sweep needs ny <= " +
+                                        std::to_string(kRowsMaxThreads));
+    c->shape = -1;
+    c->brick = 0;
+    c->threads = ((c->ny + 63) / 64) * 64;
+    c->dl = c->ndl = c->ko = c->dw = 1;
+    c->chunk = 1;
+    c->nchunks = c->dirs;
+    c->exact = c->tuning.exact_arith ? 1 : 0;
+    c->schedule = 2;
+    c->ndirsets = c->noct * c->dirs;
+    c->launches_per_sweep = 1;
+    c->ntasks = 0;
+    return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
+  }
+  if (c->dims == 3 && c->tuning.schedule == 2)
+    return fail(DOMA_ERR_INVALID, "schedule 2 (one lane per row) is a 2D sweep");
   const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
   const int shape = select_shape(c->dims, c->noct, c->dirs, minext, c->tuning);
   ShapeInfo si{};
@@ -420,6 +444,27 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.part = c->d_part;
     a.skip = c->si_device ? c->d_si_state : nullptr;
     return launch_sweep1d(a, c->precision, c->stream);
+  }
+  if (c->schedule == 2)
+  {
+    Rows2DArgs r{};
+    r.nx = c->nx;
+    r.ny = c->ny;
+    r.N = c->N;
+    r.dirs = c->dirs;
+    r.dirtab = c->d_dirtab;
+    r.lx = c->d_lx;
+    r.ly = c->d_ly;
+    r.q = q;
+    r.sig = c->d_sig;
+    r.part = c->d_part;
+    r.skip = c->si_device ? c->d_si_state : nullptr;
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
+    int rc = launch_sweep2d_rows(r, c->scheme, c->precision, c->exact, c->stream);
+    if (c->profiling)
+      DOMA_HIP(hipEventRecord(c->prof_events[1], c->stream));
+    return rc;
   }
   BrickArgs a = brick_args(c, q);
   a.skip = c->si_device ? c->d_si_state : nullptr;
@@ -1726,6 +1771,12 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
         if (select_shape(c->dims, c->noct, c->dirs, minext, t) == s)
           cands.push_back(t);
       }
+    if (c->dims == 2 && rows2d_fits(c->nx, c->ny))
+    {
+      doma_tuning t{};
+      t.schedule = 2;
+      cands.push_back(t);
+    }
   }
   const doma_tuning keep = c->tuning;
   doma_tuning win = keep;

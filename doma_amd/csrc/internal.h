@@ -100,6 +100,23 @@ struct Sweep1DArgs
   const int * skip;       // as BrickArgs::skip
 };
 
+// 2D sweep with one lane per mesh row (sweep_rows.hip), schedule 2.
+constexpr int kRowsMaxThreads = 512;  // rows per work-group (one lane each)
+struct Rows2DArgs
+{
+  int nx, ny, N;
+  int dirs;                // directions per quadrant
+  const double * dirtab;   // [4*dirs][4]: |mu|, |eta|, |xi|, w
+  const double * lx;       // [nx]
+  const double * ly;       // [ny]
+  const double * q;        // [N]
+  const double * sig;      // [N]
+  double * part;           // [4*dirs][N] per-direction w psi
+  const int * skip;        // as BrickArgs::skip
+};
+bool rows2d_fits(int nx, int ny);
+int launch_sweep2d_rows(const Rows2DArgs & a, int scheme, int precision, int exact, hipStream_t s);
+
 // Kernel shapes of the brick sweep (table in sweep_impl.h).
 struct ShapeInfo
 {

@@ -68,13 +68,36 @@ dispatch(const BrickArgs & a, int scheme, int precision, int exact, int shape, i
 }
 
 // 1D sweep: one thread per direction, cells in sweep order
-// (TransportSolver1D.cpp:840-915).
-template <typename R>
+// (TransportSolver1D.cpp:840-915).  With STAGE the slab's source, cross section
+// and widths are first copied to LDS, so each cell step reads LDS instead of
+// waiting on global memory.
+constexpr int kStage1D = 2048;  // cells staged in LDS (3 x 8 B each)
+
+template <typename R, bool STAGE>
 __global__ __launch_bounds__(1024) void
 sweep1d_kernel(Sweep1DArgs a)
 {
+  if (a.skip && *a.skip)
+    return;
+  __shared__ double s_buf[STAGE ? 3 * kStage1D : 1];
+  const double * q = a.q;
+  const double * sg = a.sig;
+  const double * lx = a.lx;
+  if constexpr (STAGE)
+  {
+    for (int i = threadIdx.x; i < a.nx; i += blockDim.x)
+    {
+      s_buf[i] = a.q[i];
+      s_buf[kStage1D + i] = a.sig[i];
+      s_buf[2 * kStage1D + i] = a.lx[i];
+    }
+    __syncthreads();
+    q = s_buf;
+    sg = s_buf + kStage1D;
+    lx = s_buf + 2 * kStage1D;
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * a.dirs || (a.skip && *a.skip))
+  if (t >= 2 * a.dirs)
     return;
   const int half = t / a.dirs;
   const double * d = a.dirtab + 4 * t;
@@ -84,7 +107,7 @@ sweep1d_kernel(Sweep1DArgs a)
   for (int ii = 0; ii < a.nx; ++ii)
   {
     const int i = half == 0 ? ii : a.nx - 1 - ii;
-    const R psi = dd1d<R>(R(a.q[i]), R(a.sig[i]), R(a.lx[i]), amu, f);
+    const R psi = dd1d<R>(R(q[i]), R(sg[i]), R(lx[i]), amu, f);
     a.part[(size_t)t * a.nx + i] = w * static_cast<double>(psi);
   }
 }
@@ -161,10 +184,18 @@ launch_sweep1d(const Sweep1DArgs & a, int precision, hipStream_t s)
   const int nthreads = 2 * a.dirs;
   const int block = nthreads <= 1024 ? ((nthreads + 63) / 64) * 64 : 1024;
   const int grid = (nthreads + block - 1) / block;
+  const bool stage = a.nx <= kStage1D;
   if (precision == DOMA_FP32_SWEEP)
-    hipLaunchKernelGGL(sweep1d_kernel<float>, dim3(grid), dim3(block), 0, s, a);
+  {
+    if (stage)
+      hipLaunchKernelGGL((sweep1d_kernel<float, true>), dim3(grid), dim3(block), 0, s, a);
+    else
+      hipLaunchKernelGGL((sweep1d_kernel<float, false>), dim3(grid), dim3(block), 0, s, a);
+  }
+  else if (stage)
+    hipLaunchKernelGGL((sweep1d_kernel<double, true>), dim3(grid), dim3(block), 0, s, a);
   else
-    hipLaunchKernelGGL(sweep1d_kernel<double>, dim3(grid), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((sweep1d_kernel<double, false>), dim3(grid), dim3(block), 0, s, a);
   return DOMA_OK;
 }
 } // namespace doma

@@ -84,8 +84,10 @@ typedef struct doma_tuning {
   int32_t threads;        /* work-group size of the sweep kernel */
   int32_t exact_arith;    /* 1: reference expression order with IEEE divides,
                              0: hoisted reciprocals (default) */
-  int32_t schedule;       /* 0: one launch per brick diagonal (default);
-                             1: one persistent dataflow launch per sweep */
+  int32_t schedule;       /* 0: one launch per brick diagonal (default in 3D);
+                             1: one persistent dataflow launch per sweep;
+                             2: 2D only, one lane per mesh row, one work-group per
+                                direction (default in 2D up to 512 rows) */
   int32_t octants;        /* octants (2D: quadrants) per work-group: 1, 2, 4 or 8 */
   int32_t dirs_per_lane;  /* directions per lane: 1 or 2 */
   int32_t shape;          /* explicit kernel shape id + 1 (overrides the hints) */

@@ -172,3 +172,59 @@ def test_fp32_sweep_error_statement(name):
     l2 = np.sqrt(((b - a) ** 2).sum(axis=1) / np.maximum((a ** 2).sum(axis=1), 1e-300))
     print(f"FP32-sweep vs FP64 {name}: max rel per group {err}, L2 rel {l2}", flush=True)
     assert np.all(err < 1e-4)
+
+
+# 2D sweep with one lane per mesh row (schedule 2): rows below, at and above a
+# wave, up to the 512-row limit; uneven widths; both schemes and arithmetics.
+ROWS_CASES = [
+    # nx, ny, np, na, scheme
+    (21, 18, 3, 2, capi.SCHEME_TW_DD),
+    (37, 64, 4, 4, capi.SCHEME_DD),
+    (5, 65, 2, 2, capi.SCHEME_TW_DD),
+    (130, 1, 2, 3, capi.SCHEME_TW_DD),
+    (1, 40, 2, 2, capi.SCHEME_DD),
+    (64, 200, 3, 3, capi.SCHEME_TW_DD),
+    (33, 512, 2, 2, capi.SCHEME_TW_DD),
+]
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+@pytest.mark.parametrize("case", ROWS_CASES, ids=lambda s: f"{s[0]}x{s[1]}_{s[2]}x{s[3]}_s{s[4]}")
+def test_rows2d_sweep_matches_oracle(case, exact):
+    nx, ny, np_, na, scheme = case
+    p = _random_problem(2, nx, ny, 1, 2, np_, na, scheme, seed=nx * 7 + ny)
+    rng = np.random.default_rng(9)
+    q = rng.uniform(0.0, 1.0, p.N) / (2 * np.pi)
+    q[rng.uniform(size=p.N) < 0.3] = 0.0
+    print(f"rows sweep {case} exact={exact}", flush=True)
+    with Context(p, tuning=dict(schedule=2, exact_arith=exact)) as ctx:
+        assert ctx.get_tuning()["schedule"] == 2
+        for g in range(2):
+            got = ctx.sweep(g, q)
+            ref = ob.sweep(p, g, q, nthreads=1)
+            assert rel_err(got[None], ref[None])[0] <= 1e-13
+            if exact:
+                assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < FP64_TOL
+
+
+def test_rows2d_default_and_limits():
+    """2D problems without an explicit tuning use the row schedule up to 512
+    rows and the brick kernel above; the row schedule is refused in 3D and
+    beyond 512 rows; solves agree with the brick kernel's."""
+    small = _random_problem(2, 40, 30, 1, 2, 2, 2, capi.SCHEME_TW_DD, seed=3)
+    with Context(small) as ctx:
+        assert ctx.get_tuning()["schedule"] == 2
+        phi_rows, st_rows = ctx.solve_fixed_source(1e-10, 500, 1e-10, 50)
+    with Context(small, tuning=dict(schedule=1)) as ctx:
+        phi_brick, st_brick = ctx.solve_fixed_source(1e-10, 500, 1e-10, 50)
+    assert st_rows["si_iterations"] == st_brick["si_iterations"]
+    assert rel_err(phi_rows, phi_brick).max() < 1e-12
+    tall = _random_problem(2, 4, 600, 1, 1, 2, 2, capi.SCHEME_DD, seed=4)
+    with Context(tall) as ctx:
+        assert ctx.get_tuning()["schedule"] != 2
+        with pytest.raises(capi.DomaError):
+            ctx.set_tuning(schedule=2)
+    cube = _random_problem(3, 6, 6, 6, 1, 2, 2, capi.SCHEME_DD, seed=5)
+    with Context(cube) as ctx:
+        with pytest.raises(capi.DomaError):
+            ctx.set_tuning(schedule=2)
