@@ -239,8 +239,10 @@ int doma_profile_sweeps(doma_ctx *ctx, int32_t g, int32_t n, double *kernel_ms, 
 /* Number of cell-direction units one sweep of one group processes. */
 int64_t doma_units_per_sweep(const doma_ctx *ctx);
 
-/* Project-written autotuner: times the candidate kernel shapes on this
- * context's problem and keeps the fastest (cached per problem shape). */
+/* Project-written autotuner: times the candidate kernel shapes and schedules
+ * on this context's problem (each sweep followed by the sum of its
+ * direction-set partials, as in an iteration) and keeps the fastest (cached per
+ * problem shape); *best_ms is that per-iteration time. */
 int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
 
 /* ------------------------------------------------------------------------- */
