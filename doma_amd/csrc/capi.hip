@@ -459,6 +459,7 @@ run_sweep_with(doma_ctx * c, const double * q)
     r.sig = c->d_sig;
     r.part = c->d_part;
     r.skip = c->si_device ? c->d_si_state : nullptr;
+    r.error = c->d_err;
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
     int rc = launch_sweep2d_rows(r, c->scheme, c->precision, c->exact, c->stream);

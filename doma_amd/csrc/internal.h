@@ -113,7 +113,9 @@ struct Rows2DArgs
   const double * sig;      // [N]
   double * part;           // [4*dirs][N] per-direction w psi
   const int * skip;        // as BrickArgs::skip
+  int * error;             // set if a wave-to-wave wait exceeds its bound
 };
+size_t rows2d_lds(int nx, int ny, size_t rs);
 bool rows2d_fits(int nx, int ny);
 int launch_sweep2d_rows(const Rows2DArgs & a, int scheme, int precision, int exact, hipStream_t s);
 

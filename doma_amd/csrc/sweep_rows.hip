@@ -6,9 +6,11 @@
 // nx + ny - 1 dependent cell solves (TransportSolver2D.cpp:635-725 visits the
 // same cells row by row).  The incoming x face stays in a register; the
 // incoming y face is the one the lane below produced a step earlier, passed by a
-// wave shift and, across waves, through LDS.  The inputs of a lane's next eight
-// cells are loaded eight steps ahead, and the column widths sit in LDS, so a
-// step costs a cell solve, a shift and a barrier rather than a global-memory
+// wave shift, and across waves through an LDS column of faces with a ready
+// count, so waves run their own step windows without a block barrier (a wave
+// only waits for the one below it, which never waits for it).  The inputs of a
+// lane's next eight cells are loaded eight steps ahead and the column widths sit
+// in LDS, so a step costs a cell solve and a shift rather than a global-memory
 // round trip.  Each work-group writes its direction's w psi into its own
 // partial-flux plane, summed in direction order by the finish kernel.
 #include <hip/hip_runtime.h>
@@ -54,8 +56,13 @@ sweep2d_rows_kernel(Rows2DArgs a)
   if (a.skip && *a.skip)
     return;
   extern __shared__ __align__(16) unsigned char smem[];
-  double * s_lx = reinterpret_cast<double *>(smem);          // [nx] in sweep order
-  R * s_y = reinterpret_cast<R *>(smem + 8 * (size_t)a.nx);  // [2][waves] wave-edge y faces
+  const int nwaves = (int)(blockDim.x >> 6);
+  double * s_lx = reinterpret_cast<double *>(smem);  // [nx] in sweep order
+  // y faces a wave's top lane hands to the next wave, by column, and how many
+  // columns of them are ready.
+  R * s_y = reinterpret_cast<R *>(smem + 8 * (size_t)a.nx);
+  int * s_ready = reinterpret_cast<int *>(smem + 8 * (size_t)a.nx +
+                                          sizeof(R) * (size_t)a.nx * (nwaves - 1));
   const int slot = blockIdx.x / a.dirs;  // quadrant: bit 2 = -mu, bit 1 = -eta
   const bool rev_x = (slot & 2) != 0, rev_y = (slot & 1) != 0;
   const double * d = a.dirtab + 4 * blockIdx.x;
@@ -63,6 +70,8 @@ sweep2d_rows_kernel(Rows2DArgs a)
   const double w = d[3];
   for (int ii = threadIdx.x; ii < a.nx; ii += blockDim.x)
     s_lx[ii] = a.lx[rev_x ? a.nx - 1 - ii : ii];
+  if (threadIdx.x < 16)
+    s_ready[threadIdx.x] = 0;
   const int jj = threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool row = jj < a.ny;
   const int j = rev_y ? a.ny - 1 - jj : jj;
@@ -72,57 +81,97 @@ sweep2d_rows_kernel(Rows2DArgs a)
   double * prow = a.part + (size_t)blockIdx.x * a.N + (size_t)(row ? j : 0) * a.nx;
   auto col = [&](int ii) { return rev_x ? a.nx - 1 - ii : ii; };
   auto clampc = [&](int ii) { return col(ii < 0 ? 0 : (ii >= a.nx ? a.nx - 1 : ii)); };
-  // Inputs of the cells of steps s0 .. s0+7 (cur) and of the next eight (nxt).
+  // A wave works the steps in which one of its rows has a column to solve:
+  // s = 64 wave .. 64 wave + (its rows - 1) + nx - 1, ahead of the waves above.
+  const int wave_rows = min(64, a.ny - 64 * wave);
+  const int s_first = 64 * wave, s_last = 64 * wave + wave_rows - 1 + a.nx - 1;
   double cq[kAhead], cs[kAhead], nq[kAhead], ns[kAhead];
 #pragma unroll
   for (int u = 0; u < kAhead; ++u)
   {
-    const int c = clampc(u - jj);
+    const int c = clampc(s_first + u - jj);
     cq[u] = qrow[c];
     cs[u] = srow[c];
   }
   __syncthreads();
+  const R * y_below = s_y + (size_t)(wave - 1) * a.nx;  // wave > 0 only
+  R * y_above = s_y + (size_t)wave * a.nx;             // wave < nwaves - 1 only
+  int spins = 0, published = 0;
   R x = R(0), yo = R(0);
-  const int nsteps = a.nx + a.ny - 1;
-  for (int s0 = 0; s0 < nsteps; s0 += kAhead)
-  {
+  // Eight steps on the inputs in (uq, us) while the next eight cells' inputs
+  // load into (lq, ls); two register sets alternate, so no copy waits on them.
+  auto group = [&](int s0, const double (&uq)[kAhead], const double (&us)[kAhead],
+                   double (&lq)[kAhead], double (&ls)[kAhead]) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
     {
       const int c = clampc(s0 + kAhead + u - jj);
-      nq[u] = qrow[c];
-      ns[u] = srow[c];
+      lq[u] = qrow[c];
+      ls[u] = srow[c];
     }
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
     {
       const int s = s0 + u;
-      if (s >= nsteps)
+      if (s > s_last)
         break;
-      // y face from the lane below, produced at step s - 1.
+      // y face from the lane below, produced at step s - 1; lane 0 takes the
+      // wave below's top-row face of the same column, once it is ready (the
+      // wave below never waits on this one, so the wait ends).
       R yin = __shfl_up(yo, 1, 64);
-      if (lane == 0)
-        yin = wave > 0 && s > 0 ? s_y[((s - 1) & 1) * 16 + wave - 1] : R(0);
+      const int ii0 = s - 64 * wave;  // lane 0's column (uniform)
+      if (wave > 0 && ii0 >= 0 && ii0 < a.nx)
+      {
+        while (__hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) <= ii0)
+        {
+          if (++spins > (1 << 26))
+          {
+            if (lane == 0)
+              __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");  // LDS only
+        const R yb = y_below[ii0];
+        if (lane == 0)
+          yin = yb;
+      }
       if (jj == 0)
         yin = R(0);
       const int ii = s - jj;
       if (row && ii >= 0 && ii < a.nx)
       {
         R y = yin;
-        const R psi = cell2d<R, SCHEME, EXACT>(cq[u], cs[u], s_lx[ii], ly, amu, aeta, axi, x, y);
+        const R psi = cell2d<R, SCHEME, EXACT>(uq[u], us[u], s_lx[ii], ly, amu, aeta, axi, x, y);
         yo = y;
         prow[col(ii)] = w * static_cast<double>(psi);
       }
-      if (lane == 63)
-        s_y[(s & 1) * 16 + wave] = yo;
-      __syncthreads();
+      // The top lane leaves its face for the wave above.
+      const int ii63 = s - 64 * wave - 63;
+      if (wave < nwaves - 1 && ii63 >= 0 && ii63 < a.nx)
+      {
+        if (lane == 63)
+          y_above[ii63] = yo;
+        published = ii63 + 1;
+      }
     }
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u)
+    // Once per group: order the group's faces before their count (LDS only: the
+    // flux stores to global memory are not waited for), then publish the count.
+    // The wave above starts 64 steps behind, so the batching does not delay it.
+    if (wave < nwaves - 1 && published > 0)
     {
-      cq[u] = nq[u];
-      cs[u] = ns[u];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      if (lane == 63)
+        __hip_atomic_store(s_ready + wave, published, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+  };
+  for (int s0 = s_first; s0 <= s_last; s0 += 2 * kAhead)
+  {
+    group(s0, cq, cs, nq, ns);
+    if (s0 + kAhead <= s_last)
+      group(s0 + kAhead, nq, ns, cq, cs);
   }
 }
 
@@ -131,17 +180,25 @@ int
 launch_rows(const Rows2DArgs & a, hipStream_t s)
 {
   const int threads = ((a.ny + 63) / 64) * 64;
-  const size_t lds = 8 * (size_t)a.nx + 2 * 16 * sizeof(R);
+  const size_t lds = rows2d_lds(a.nx, a.ny, sizeof(R));
   hipLaunchKernelGGL((sweep2d_rows_kernel<R, SCHEME, EXACT>), dim3(4 * a.dirs), dim3(threads),
                      lds, s, a);
   return DOMA_OK;
 }
 } // namespace
 
+// LDS: column widths, the wave-to-wave face columns, 16 ready counters.
+size_t
+rows2d_lds(int nx, int ny, size_t rs)
+{
+  const int nwaves = (ny + 63) / 64;
+  return 8 * (size_t)nx + rs * (size_t)nx * (nwaves - 1) + 16 * sizeof(int);
+}
+
 bool
 rows2d_fits(int nx, int ny)
 {
-  return ny >= 1 && ny <= kRowsMaxThreads && nx >= 1 && 8 * (size_t)nx + 256 <= 64 * 1024;
+  return ny >= 1 && ny <= kRowsMaxThreads && nx >= 1 && rows2d_lds(nx, ny, 8) <= 64 * 1024;
 }
 
 int
@@ -149,7 +206,8 @@ launch_sweep2d_rows(const Rows2DArgs & a, int scheme, int precision, int exact, 
 {
   if (!rows2d_fits(a.nx, a.ny))
     return fail(DOMA_ERR_INVALID, "row-pipelined 2D sweep needs ny <= " +
-                                      std::to_string(kRowsMaxThreads) + " and nx <= 8160");
+                                      std::to_string(kRowsMaxThreads) +
+                                      " and nx * ny within its LDS budget");
   const bool f32 = precision == DOMA_FP32_SWEEP;
   if (scheme == DOMA_SCHEME_TW_DD)
   {
