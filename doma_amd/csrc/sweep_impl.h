@@ -281,80 +281,132 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   // The previous task of this work-group may still be reading LDS.
   __syncthreads();
 
-  // Inflow faces of every octant: the upwind brick's outflow, or vacuum.
-  for (int e = tid; e < KO * NAX * FS * DW; e += NT)
+  // Inflow faces of every octant (the upwind brick's outflow, or vacuum) and the
+  // cell inputs of every octant's cells: each lane issues all of its loads of a
+  // group before waiting for any, so a task's set-up costs one memory latency
+  // per group instead of one per element.
+  constexpr int NF = KO * NAX * FS * DW;   // face values
+  constexpr int FI = (NF + NT - 1) / NT;   // per lane
+  constexpr int FG = FI < 4 ? FI : 4;      // loads in flight per group
+#pragma unroll
+  for (int f0 = 0; f0 < FI; f0 += FG)
   {
-    const int k_o = e / (NAX * FS * DW);
-    const int r = e - k_o * (NAX * FS * DW);
-    const int ax = r / (FS * DW);
-    const int s = (r / DW) % FS, l = r % DW;
-    const int u = DIM == 3 ? s / B : s;
-    const int v = DIM == 3 ? s % B : 0;
-    const int dsk = (og * KO + k_o) * a.nchunks + ch;
-    R val = R(0);
-    if (ax == 0 && bi > 0)
-      val = face_load<COHERENT>(gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp +
-                                                     (size_t)(j0 + u) * a.nzp + (k0 + v))
-                                                  : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l);
-    else if (ax == 1 && bj > 0)
-      val = face_load<COHERENT>(gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp +
-                                                     (size_t)(i0 + u) * a.nzp + (k0 + v))
-                                                  : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l);
-    else if (ax == 2 && bk > 0)
-      val = face_load<COHERENT>(gf[2] + ((size_t)dsk * a.nxp * a.nyp + (size_t)(i0 + u) * a.nyp +
-                                         (j0 + v)) * DW + l);
-    faces[e] = val;
-  }
-  // Cell records of every octant, brick-local order (lk*B + lj)*B + li.
-  for (int e = tid; e < KO * NC; e += NT)
-  {
-    const int k_o = e / NC, ce = e - k_o * NC;
-    bool nx, ny, nz;
-    orient<DIM>(og * KO + k_o, nx, ny, nz);
-    const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
-    const int oi = i0 + li, oj = j0 + lj, ok = k0 + lk;
-    CellRec r;
-    r.v0 = 0.0;
-    r.v1 = 1.0;
-    r.g0 = r.g1 = r.g2 = 1.0;
-    r.c = -1;
-    r.pad = 0;
-    if (oi < a.nx && oj < a.ny && ok < a.nz)
+    R val[FG];
+#pragma unroll
+    for (int k = 0; k < FG; ++k)
     {
-      const int i = nx ? a.nx - 1 - oi : oi;
-      const int j = ny ? a.ny - 1 - oj : oj;
-      const int k = nz ? a.nz - 1 - ok : ok;
-      const int c = (k * a.ny + j) * a.nx + i;
-      const double q = a.q[c], sg = a.sig[c];
-      const double lx = a.lx[i], ly = a.ly[j], lz = DIM == 3 ? a.lz[k] : 1.0;
-      r.c = c;
-      if (EXACT)
+      const int e = tid + (f0 + k) * NT;
+      val[k] = R(0);
+      if (f0 + k < FI && e < NF)
       {
-        r.v0 = q;
-        r.v1 = sg;
-        r.g0 = lx;
-        r.g1 = ly;
-        r.g2 = lz;
-      }
-      else if (SCHEME == DOMA_SCHEME_DD)
-      {
-        r.v0 = q;
-        r.v1 = sg;
-        r.g0 = 1.0 / lx;
-        r.g1 = 1.0 / ly;
-        r.g2 = 1.0 / lz;
-      }
-      else
-      {
-        const double V = DIM == 3 ? a.lx[i] * a.ly[j] * a.lz[k] : a.lx[i] * a.ly[j];
-        r.v0 = q * V;
-        r.v1 = sg * V;
-        r.g0 = DIM == 3 ? ly * lz : ly;
-        r.g1 = DIM == 3 ? lx * lz : lx;
-        r.g2 = DIM == 3 ? lx * ly : 1.0;
+        const int k_o = e / (NAX * FS * DW);
+        const int r = e - k_o * (NAX * FS * DW);
+        const int ax = r / (FS * DW);
+        const int s = (r / DW) % FS, l = r % DW;
+        const int u = DIM == 3 ? s / B : s;
+        const int v = DIM == 3 ? s % B : 0;
+        const int dsk = (og * KO + k_o) * a.nchunks + ch;
+        if (ax == 0 && bi > 0)
+          val[k] = face_load<COHERENT>(gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp +
+                                                            (size_t)(j0 + u) * a.nzp + (k0 + v))
+                                                         : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l);
+        else if (ax == 1 && bj > 0)
+          val[k] = face_load<COHERENT>(gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp +
+                                                            (size_t)(i0 + u) * a.nzp + (k0 + v))
+                                                         : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l);
+        else if (ax == 2 && bk > 0)
+          val[k] = face_load<COHERENT>(gf[2] + ((size_t)dsk * a.nxp * a.nyp +
+                                                (size_t)(i0 + u) * a.nyp + (j0 + v)) * DW + l);
       }
     }
-    rec[e] = r;
+#pragma unroll
+    for (int k = 0; k < FG; ++k)
+    {
+      const int e = tid + (f0 + k) * NT;
+      if (f0 + k < FI && e < NF)
+        faces[e] = val[k];
+    }
+  }
+  // Cell records of every octant, brick-local order (lk*B + lj)*B + li.
+  constexpr int NR = KO * NC;             // records
+  constexpr int RI = (NR + NT - 1) / NT;  // per lane
+  constexpr int RG = RI < 4 ? RI : 4;     // cells whose inputs are in flight per group
+#pragma unroll
+  for (int r0 = 0; r0 < RI; r0 += RG)
+  {
+    double q[RG], sg[RG], lx[RG], ly[RG], lz[RG];
+    int cell[RG];
+#pragma unroll
+    for (int k = 0; k < RG; ++k)
+    {
+      const int e = tid + (r0 + k) * NT;
+      cell[k] = -1;
+      q[k] = 0.0;
+      sg[k] = lx[k] = ly[k] = lz[k] = 1.0;
+      if (r0 + k < RI && e < NR)
+      {
+        const int k_o = e / NC, ce = e - k_o * NC;
+        bool nx, ny, nz;
+        orient<DIM>(og * KO + k_o, nx, ny, nz);
+        const int li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
+        const int oi = i0 + li, oj = j0 + lj, ok = k0 + lk;
+        if (oi < a.nx && oj < a.ny && ok < a.nz)
+        {
+          const int i = nx ? a.nx - 1 - oi : oi;
+          const int j = ny ? a.ny - 1 - oj : oj;
+          const int k3 = nz ? a.nz - 1 - ok : ok;
+          const int c = (k3 * a.ny + j) * a.nx + i;
+          cell[k] = c;
+          q[k] = a.q[c];
+          sg[k] = a.sig[c];
+          lx[k] = a.lx[i];
+          ly[k] = a.ly[j];
+          lz[k] = DIM == 3 ? a.lz[k3] : 1.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RG; ++k)
+    {
+      const int e = tid + (r0 + k) * NT;
+      if (!(r0 + k < RI && e < NR))
+        continue;
+      CellRec r;
+      r.v0 = 0.0;
+      r.v1 = 1.0;
+      r.g0 = r.g1 = r.g2 = 1.0;
+      r.c = cell[k];
+      r.pad = 0;
+      if (cell[k] >= 0)
+      {
+        if (EXACT)
+        {
+          r.v0 = q[k];
+          r.v1 = sg[k];
+          r.g0 = lx[k];
+          r.g1 = ly[k];
+          r.g2 = lz[k];
+        }
+        else if (SCHEME == DOMA_SCHEME_DD)
+        {
+          r.v0 = q[k];
+          r.v1 = sg[k];
+          r.g0 = 1.0 / lx[k];
+          r.g1 = 1.0 / ly[k];
+          r.g2 = 1.0 / lz[k];
+        }
+        else
+        {
+          const double V = DIM == 3 ? lx[k] * ly[k] * lz[k] : lx[k] * ly[k];
+          r.v0 = q[k] * V;
+          r.v1 = sg[k] * V;
+          r.g0 = DIM == 3 ? ly[k] * lz[k] : ly[k];
+          r.g1 = DIM == 3 ? lx[k] * lz[k] : lx[k];
+          r.g2 = DIM == 3 ? lx[k] * ly[k] : 1.0;
+        }
+      }
+      rec[e] = r;
+    }
   }
   __syncthreads();
 
