@@ -231,10 +231,17 @@ face_store(R * p, R v)
 // Brick body                                                                 //
 // ------------------------------------------------------------------------- //
 
+// No upwind dependency to wait for (the diagonal schedule orders bricks by launch).
+struct NoWait
+{
+  __device__ void operator()() const {}
+};
+
 template <typename R, int DIM, int SCHEME, bool EXACT, int B, int DL, int NDL, int KO, int NT,
-          bool COHERENT>
+          bool COHERENT, typename Wait = NoWait>
 __device__ __forceinline__ void
-sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigned char * smem)
+sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigned char * smem,
+            Wait wait_upwind = Wait{})
 {
   using L = Layout<R, DIM, B, DL, NDL, KO>;
   constexpr int DW = L::DW, FS = L::FS, NC = L::NC, NAX = L::NAX;
@@ -281,61 +288,17 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   // The previous task of this work-group may still be reading LDS.
   __syncthreads();
 
-  // Inflow faces of every octant (the upwind brick's outflow, or vacuum) and the
-  // cell inputs of every octant's cells: each lane issues all of its loads of a
-  // group before waiting for any, so a task's set-up costs one memory latency
-  // per group instead of one per element.
-  constexpr int NF = KO * NAX * FS * DW;   // face values
-  constexpr int FI = (NF + NT - 1) / NT;   // per lane
-  constexpr int FG = FI < 4 ? FI : 4;      // loads in flight per group
-#pragma unroll
-  for (int f0 = 0; f0 < FI; f0 += FG)
-  {
-    R val[FG];
-#pragma unroll
-    for (int k = 0; k < FG; ++k)
-    {
-      const int e = tid + (f0 + k) * NT;
-      val[k] = R(0);
-      if (f0 + k < FI && e < NF)
-      {
-        const int k_o = e / (NAX * FS * DW);
-        const int r = e - k_o * (NAX * FS * DW);
-        const int ax = r / (FS * DW);
-        const int s = (r / DW) % FS, l = r % DW;
-        const int u = DIM == 3 ? s / B : s;
-        const int v = DIM == 3 ? s % B : 0;
-        const int dsk = (og * KO + k_o) * a.nchunks + ch;
-        if (ax == 0 && bi > 0)
-          val[k] = face_load<COHERENT>(gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp +
-                                                            (size_t)(j0 + u) * a.nzp + (k0 + v))
-                                                         : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l);
-        else if (ax == 1 && bj > 0)
-          val[k] = face_load<COHERENT>(gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp +
-                                                            (size_t)(i0 + u) * a.nzp + (k0 + v))
-                                                         : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l);
-        else if (ax == 2 && bk > 0)
-          val[k] = face_load<COHERENT>(gf[2] + ((size_t)dsk * a.nxp * a.nyp +
-                                                (size_t)(i0 + u) * a.nyp + (j0 + v)) * DW + l);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < FG; ++k)
-    {
-      const int e = tid + (f0 + k) * NT;
-      if (f0 + k < FI && e < NF)
-        faces[e] = val[k];
-    }
-  }
-  // Cell records of every octant, brick-local order (lk*B + lj)*B + li.
+  // Cell records of every octant, brick-local order (lk*B + lj)*B + li, and the
+  // inflow faces of every octant (the upwind brick's outflow, or vacuum).  Each
+  // lane issues its loads in groups before waiting for any: the first group of
+  // cell inputs (which no upwind task produces) before the wait for the upwind
+  // tasks, then the faces, so one memory latency covers both.
   constexpr int NR = KO * NC;             // records
   constexpr int RI = (NR + NT - 1) / NT;  // per lane
   constexpr int RG = RI < 4 ? RI : 4;     // cells whose inputs are in flight per group
-#pragma unroll
-  for (int r0 = 0; r0 < RI; r0 += RG)
-  {
-    double q[RG], sg[RG], lx[RG], ly[RG], lz[RG];
-    int cell[RG];
+  auto load_cells = [&](int r0, double (&q)[RG], double (&sg)[RG], double (&lx)[RG],
+                        double (&ly)[RG], double (&lz)[RG], int (&cell)[RG])
+                        __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < RG; ++k)
     {
@@ -365,6 +328,10 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
         }
       }
     }
+  };
+  auto store_cells = [&](int r0, const double (&q)[RG], const double (&sg)[RG],
+                         const double (&lx)[RG], const double (&ly)[RG], const double (&lz)[RG],
+                         const int (&cell)[RG]) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < RG; ++k)
     {
@@ -407,6 +374,63 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
       }
       rec[e] = r;
     }
+  };
+  double cq[RG], csg[RG], clx[RG], cly[RG], clz[RG];
+  int ccell[RG];
+  load_cells(0, cq, csg, clx, cly, clz, ccell);
+
+  // Upwind tasks of this sweep have published their outflow faces.
+  wait_upwind();
+  __syncthreads();
+
+  constexpr int NF = KO * NAX * FS * DW;   // face values
+  constexpr int FI = (NF + NT - 1) / NT;   // per lane
+  constexpr int FG = FI < 4 ? FI : 4;      // loads in flight per group
+#pragma unroll
+  for (int f0 = 0; f0 < FI; f0 += FG)
+  {
+    R val[FG];
+#pragma unroll
+    for (int k = 0; k < FG; ++k)
+    {
+      const int e = tid + (f0 + k) * NT;
+      val[k] = R(0);
+      if (f0 + k < FI && e < NF)
+      {
+        const int k_o = e / (NAX * FS * DW);
+        const int r = e - k_o * (NAX * FS * DW);
+        const int ax = r / (FS * DW);
+        const int s = (r / DW) % FS, l = r % DW;
+        const int u = DIM == 3 ? s / B : s;
+        const int v = DIM == 3 ? s % B : 0;
+        const int dsk = (og * KO + k_o) * a.nchunks + ch;
+        if (ax == 0 && bi > 0)
+          val[k] = face_load<COHERENT>(gf[0] + (DIM == 3 ? ((size_t)dsk * a.nyp * a.nzp +
+                                                            (size_t)(j0 + u) * a.nzp + (k0 + v))
+                                                         : ((size_t)dsk * a.nyp + (j0 + u))) * DW + l);
+        else if (ax == 1 && bj > 0)
+          val[k] = face_load<COHERENT>(gf[1] + (DIM == 3 ? ((size_t)dsk * a.nxp * a.nzp +
+                                                            (size_t)(i0 + u) * a.nzp + (k0 + v))
+                                                         : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l);
+        else if (ax == 2 && bk > 0)
+          val[k] = face_load<COHERENT>(gf[2] + ((size_t)dsk * a.nxp * a.nyp +
+                                                (size_t)(i0 + u) * a.nyp + (j0 + v)) * DW + l);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < FG; ++k)
+    {
+      const int e = tid + (f0 + k) * NT;
+      if (f0 + k < FI && e < NF)
+        faces[e] = val[k];
+    }
+  }
+  store_cells(0, cq, csg, clx, cly, clz, ccell);
+#pragma unroll
+  for (int r0 = RG; r0 < RI; r0 += RG)
+  {
+    load_cells(r0, cq, csg, clx, cly, clz, ccell);
+    store_cells(r0, cq, csg, clx, cly, clz, ccell);
   }
   __syncthreads();
 
@@ -604,18 +628,19 @@ brick_flow_kernel(BrickArgs a)
     const int og = r / a.nchunks, ch = r % a.nchunks;
     const int bi = a.bricks[3 * br + 0], bj = a.bricks[3 * br + 1], bk = a.bricks[3 * br + 2];
     uint32_t * flags = a.flags + (size_t)r * nbricks;
-    if (threadIdx.x == 0)
-    {
-      if (bi > 0)
-        wait_flag(a, flags + (bk * a.nty + bj) * a.ntx + (bi - 1));
-      if (bj > 0)
-        wait_flag(a, flags + (bk * a.nty + (bj - 1)) * a.ntx + bi);
-      if (bk > 0)
-        wait_flag(a, flags + ((bk - 1) * a.nty + bj) * a.ntx + bi);
-    }
-    __syncthreads();
+    auto wait_upwind = [&]() {
+      if (threadIdx.x == 0)
+      {
+        if (bi > 0)
+          wait_flag(a, flags + (bk * a.nty + bj) * a.ntx + (bi - 1));
+        if (bj > 0)
+          wait_flag(a, flags + (bk * a.nty + (bj - 1)) * a.ntx + bi);
+        if (bk > 0)
+          wait_flag(a, flags + ((bk - 1) * a.nty + bj) * a.ntx + bi);
+      }
+    };
     sweep_brick<R, DIM, SCHEME, EXACT, S.B, S.DL, S.NDL, S.KO, S.NT, true>(a, bi, bj, bk, og, ch,
-                                                                           smem);
+                                                                           smem, wait_upwind);
     // Every thread's coherent face stores must have completed before the flag.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -662,18 +687,22 @@ brick_batch_kernel(BatchArgs b)
     const int og = r / a.nchunks, ch = r % a.nchunks;
     const int bi = a.bricks[3 * br + 0], bj = a.bricks[3 * br + 1], bk = a.bricks[3 * br + 2];
     uint32_t * flags = a.flags + (size_t)r * nbricks;
-    if (threadIdx.x == 0)
-    {
-      if (bi > 0)
-        wait_epoch(flags + (bk * a.nty + bj) * a.ntx + (bi - 1), b.epoch, b.error, b.wait_ticks);
-      if (bj > 0)
-        wait_epoch(flags + (bk * a.nty + (bj - 1)) * a.ntx + bi, b.epoch, b.error, b.wait_ticks);
-      if (bk > 0)
-        wait_epoch(flags + ((bk - 1) * a.nty + bj) * a.ntx + bi, b.epoch, b.error, b.wait_ticks);
-    }
-    __syncthreads();
+    auto wait_upwind = [&]() {
+      if (threadIdx.x == 0)
+      {
+        if (bi > 0)
+          wait_epoch(flags + (bk * a.nty + bj) * a.ntx + (bi - 1), b.epoch, b.error,
+                     b.wait_ticks);
+        if (bj > 0)
+          wait_epoch(flags + (bk * a.nty + (bj - 1)) * a.ntx + bi, b.epoch, b.error,
+                     b.wait_ticks);
+        if (bk > 0)
+          wait_epoch(flags + ((bk - 1) * a.nty + bj) * a.ntx + bi, b.epoch, b.error,
+                     b.wait_ticks);
+      }
+    };
     sweep_brick<R, 3, SCHEME, false, S.B, S.DL, S.NDL, S.KO, S.NT, true>(a, bi, bj, bk, og, ch,
-                                                                         smem);
+                                                                         smem, wait_upwind);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0)
