@@ -145,9 +145,11 @@ shape_lds_bytes(int shape, int precision)
   const int nc = si.dims == 3 ? si.brick * si.brick * si.brick : si.brick * si.brick;
   const int nax = si.dims == 3 ? 3 : 2;
   const int nplanes = si.dims == 3 ? 3 * si.brick - 2 : 2 * si.brick - 1;
-  // faces, cell records (48 B), plane table, plane offsets, task slot
-  return (rs * fs * si.dl * si.ndl * nax * si.ko + 15) / 16 * 16 + 48 * (size_t)nc * si.ko +
-         4 * (size_t)nc + 4 * (size_t)(nplanes + 1) + 16;
+  // faces, cell records (48 B), plane table, plane offsets, inverse plane table,
+  // task slot
+  const size_t pi_off = (rs * fs * si.dl * si.ndl * nax * si.ko + 15) / 16 * 16 +
+                        48 * (size_t)nc * si.ko + 4 * (size_t)nc + 4 * (size_t)(nplanes + 1);
+  return (pi_off + 2 * (size_t)nc + 15) / 16 * 16 + 16;
 }
 
 int

@@ -164,7 +164,8 @@ struct Layout
   static constexpr size_t REC_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;   // CellRec [KO][NC]
   static constexpr size_t PT_OFF = REC_OFF + sizeof(CellRec) * NC * KO;  // uint32 [NC]
   static constexpr size_t PO_OFF = PT_OFF + 4 * NC;                        // int [NPLANES+1]
-  static constexpr size_t TASK_OFF = PO_OFF + 4 * (NPLANES + 1);
+  static constexpr size_t PI_OFF = PO_OFF + 4 * (NPLANES + 1);                // uint16 [NC]
+  static constexpr size_t TASK_OFF = (PI_OFF + 2 * NC + 15) / 16 * 16;
   static constexpr size_t BYTES = TASK_OFF + 16;
 };
 
@@ -179,9 +180,11 @@ load_plane_table(const BrickArgs & a, unsigned char * smem)
   static_assert(L::NC <= 512 && L::FS <= 64, "plane table packing");
   uint32_t * pt = reinterpret_cast<uint32_t *>(smem + L::PT_OFF);
   int32_t * po = reinterpret_cast<int32_t *>(smem + L::PO_OFF);
+  uint16_t * pi = reinterpret_cast<uint16_t *>(smem + L::PI_OFF);
   for (int e = threadIdx.x; e < L::NC; e += NT)
   {
     const uint32_t ce = a.plane_cells[e];
+    pi[ce] = (uint16_t)e;
     const uint32_t li = ce % B, lj = (ce / B) % B, lk = DIM == 3 ? ce / (B * B) : 0;
     const uint32_t sx = DIM == 3 ? lj * B + lk : lj;
     const uint32_t sy = DIM == 3 ? li * B + lk : li;
@@ -253,6 +256,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   CellRec * rec = reinterpret_cast<CellRec *>(smem + L::REC_OFF);
   const uint32_t * pt = reinterpret_cast<const uint32_t *>(smem + L::PT_OFF);
   const int32_t * po = reinterpret_cast<const int32_t *>(smem + L::PO_OFF);
+  const uint16_t * pi = reinterpret_cast<const uint16_t *>(smem + L::PI_OFF);
 
   const int tid = threadIdx.x;
   const int dd = tid % DL;
@@ -288,7 +292,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   // The previous task of this work-group may still be reading LDS.
   __syncthreads();
 
-  // Cell records of every octant, brick-local order (lk*B + lj)*B + li, and the
+  // Cell records of every octant, in plane-table order (cells are read in
+  // brick-local order ce = (lk*B + lj)*B + li and stored at position pi[ce]), and the
   // inflow faces of every octant (the upwind brick's outflow, or vacuum).  Each
   // lane issues its loads in groups before waiting for any: the first group of
   // cell inputs (which no upwind task produces) before the wait for the upwind
@@ -372,7 +377,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
           r.g2 = DIM == 3 ? lx[k] * ly[k] : 1.0;
         }
       }
-      rec[e] = r;
+      const int k_o = e / NC;
+      rec[k_o * NC + pi[e - k_o * NC]] = r;
     }
   };
   double cq[RG], csg[RG], clx[RG], cly[RG], clz[RG];
@@ -439,77 +445,102 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   R * fzo = fyo + FS * DW;
   const CellRec * rco = rec + ko * NC;
 
-  for (int p = 0; p < L::NPLANES; ++p)
-  {
-    const int pe = po[p + 1];
-    for (int pos = po[p] + cpo; pos < pe; pos += CPO)
-    {
-      // Cells of a partial brick outside the grid are solved too (on finite
-      // dummy data): they only feed other outside cells, and their scalar flux
-      // is not stored.
-      const uint32_t info = pt[pos];
-      const CellRec r = rco[info & 511u];
-      R * xf = fxo + ((info >> 9) & 63u) * DW;
-      R * yf = fyo + ((info >> 15) & 63u) * DW;
-      R * zf = fzo + ((info >> 21) & 63u) * DW;
-      double s = 0.0;
+  // One cell solve of the lane: x-, y- and z-faces in LDS updated in place, the
+  // direction-reduced scalar flux stored once per (cell, direction set).
+  auto solve = [&](uint32_t info, const CellRec & r) __attribute__((always_inline)) {
+    R * xf = fxo + ((info >> 9) & 63u) * DW;
+    R * yf = fyo + ((info >> 15) & 63u) * DW;
+    R * zf = fzo + ((info >> 21) & 63u) * DW;
+    double s = 0.0;
 #pragma unroll
-      for (int m = 0; m < NDL; ++m)
+    for (int m = 0; m < NDL; ++m)
+    {
+      R x = xf[m], y = yf[m];
+      R z = DIM == 3 ? zf[m] : R(0);
+      R psi;
+      if constexpr (DIM == 3)
       {
-        R x = xf[m], y = yf[m];
-        R z = DIM == 3 ? zf[m] : R(0);
-        R psi;
-        if constexpr (DIM == 3)
+        if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
         {
-          if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
-          {
-            if constexpr (EXACT)
-              psi = twdd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
-                                    axi[m], x, y, z);
-            else
-              psi = twdd3d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1),
-                                   axi[m] * R(r.g2), x, y, z);
-          }
-          else
-          {
-            if constexpr (EXACT)
-              psi = dd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
+          if constexpr (EXACT)
+            psi = twdd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
                                   axi[m], x, y, z);
-            else
-              psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
-                                 R(2) * aeta[m] * R(r.g1), R(2) * axi[m] * R(r.g2), x, y, z);
-          }
+          else
+            psi = twdd3d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1),
+                                 axi[m] * R(r.g2), x, y, z);
         }
         else
         {
-          if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
-          {
-            if constexpr (EXACT)
-              psi = twdd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], axi[m], x,
-                                    y);
-            else
-              psi = twdd2d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1), axi[m], x,
-                                   y);
-          }
+          if constexpr (EXACT)
+            psi = dd3d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), R(r.g2), amu[m], aeta[m],
+                                axi[m], x, y, z);
           else
-          {
-            if constexpr (EXACT)
-              psi = dd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], x, y);
-            else
-              psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
-                                 R(2) * aeta[m] * R(r.g1), R(0), x, y, z);
-          }
+            psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
+                               R(2) * aeta[m] * R(r.g1), R(2) * axi[m] * R(r.g2), x, y, z);
         }
-        xf[m] = x;
-        yf[m] = y;
-        if constexpr (DIM == 3)
-          zf[m] = z;
-        s += w[m] * static_cast<double>(psi);
       }
-      s = group_sum<DL>(s);
-      if (dd == 0 && r.c >= 0)
-        a.part[(size_t)ds * a.N + r.c] = s;
+      else
+      {
+        if constexpr (SCHEME == DOMA_SCHEME_TW_DD)
+        {
+          if constexpr (EXACT)
+            psi = twdd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], axi[m], x,
+                                  y);
+          else
+            psi = twdd2d_fast<R>(R(r.v0), R(r.v1), amu[m] * R(r.g0), aeta[m] * R(r.g1), axi[m], x,
+                                 y);
+        }
+        else
+        {
+          if constexpr (EXACT)
+            psi = dd2d_exact<R>(R(r.v0), R(r.v1), R(r.g0), R(r.g1), amu[m], aeta[m], x, y);
+          else
+            psi = dd3d_fast<R>(R(r.v0), R(r.v1), R(2) * amu[m] * R(r.g0),
+                               R(2) * aeta[m] * R(r.g1), R(0), x, y, z);
+        }
+      }
+      xf[m] = x;
+      yf[m] = y;
+      if constexpr (DIM == 3)
+        zf[m] = z;
+      s += w[m] * static_cast<double>(psi);
     }
+    s = group_sum<DL>(s);
+    if (dd == 0 && r.c >= 0)
+      a.part[(size_t)ds * a.N + r.c] = s;
+  };
+
+  // The lane's first cell of each plane: its plane-table entry and record are
+  // read at the end of the previous plane, before its barrier (neither changes
+  // during the task), so after the barrier only the face reads stand between
+  // the lane and its next solve.  Records sit in plane-table order, so the
+  // record read does not wait for the plane-table entry.  Cells of a partial
+  // brick outside the grid are solved too (on finite dummy data): they only
+  // feed other outside cells, and their scalar flux is not stored.
+  int pb = po[0] + cpo, pe = po[1];
+  uint32_t info_n = 0;
+  CellRec r_n{};
+  if (pb < pe)
+  {
+    info_n = pt[pb];
+    r_n = rco[pb];
+  }
+  for (int p = 0; p < L::NPLANES; ++p)
+  {
+    const int nb = pe + cpo, ne = p + 1 < L::NPLANES ? po[p + 2] : pe;
+    if (pb < pe)
+    {
+      solve(info_n, r_n);
+      for (int pos = pb + CPO; pos < pe; pos += CPO)
+        solve(pt[pos], rco[pos]);
+    }
+    if (p + 1 < L::NPLANES && nb < ne)
+    {
+      info_n = pt[nb];
+      r_n = rco[nb];
+    }
+    pb = nb;
+    pe = ne;
     __syncthreads();
   }
 
