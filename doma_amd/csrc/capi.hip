@@ -166,7 +166,8 @@ int
 reset_after_sweep_error(doma_ctx * c)
 {
   DOMA_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
-  DOMA_HIP(hipMemsetAsync(c->d_ticket, 0, sizeof(unsigned long long), c->stream));
+  DOMA_HIP(hipMemsetAsync(c->d_ticket, 0,
+                          sizeof(unsigned long long) * kTicketQueues * kTicketStride, c->stream));
   if (c->d_flags)
     DOMA_HIP(hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t) * (size_t)c->ntasks, c->stream));
   DOMA_HIP(hipStreamSynchronize(c->stream));
@@ -377,6 +378,9 @@ configure_sweep(doma_ctx * c)
     return rc;
   DOMA_HIP(hipMemset(c->d_flags, 0, sizeof(uint32_t) * (size_t)c->ntasks));
   c->epoch = 0;
+  // The ticket queues restart with the new task list.
+  DOMA_HIP(hipMemset(c->d_ticket, 0, sizeof(unsigned long long) * kTicketQueues * kTicketStride));
+  c->ticket_base = 0;
   if (c->schedule == 1)
   {
     int per_cu = 0;
@@ -384,7 +388,25 @@ configure_sweep(doma_ctx * c)
       return rc;
     hipDeviceProp_t prop;
     DOMA_HIP(hipGetDeviceProperties(&prop, c->device));
-    c->grid_flow = std::max(1, std::min(c->ntasks, std::max(per_cu, 1) * prop.multiProcessorCount));
+    // As many queues as divide the direction-set pairs per brick (up to 16), and
+    // a resident grid that is a multiple of them.
+    const int pairs = c->ndirsets / c->ko;
+    c->nqueues = 1;
+    for (int nq = kTicketQueues; nq > 1; nq /= 2)
+      if (pairs % nq == 0)
+      {
+        c->nqueues = nq;
+        break;
+      }
+    const int resident = std::max(per_cu, 1) * prop.multiProcessorCount;
+    int grid = std::min(c->ntasks, resident);
+    grid = std::max(c->nqueues, grid / c->nqueues * c->nqueues);
+    if (grid > resident)
+    {
+      c->nqueues = 1;
+      grid = std::max(1, std::min(c->ntasks, resident));
+    }
+    c->grid_flow = grid;
   }
   return dev_alloc(&c->d_part, (size_t)c->ndirsets * c->N);
 }
@@ -476,6 +498,7 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.epoch = ++c->epoch;
     a.ticket = c->d_ticket;
     a.ticket_base = c->ticket_base;
+    a.nqueues = c->nqueues;
     a.error = c->d_err;
     // 250 ms per dependency: orders of magnitude above any brick's latency.
     a.wait_ticks = (long long)c->wall_clock_khz * 250;
@@ -899,10 +922,12 @@ create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
       hipHostMalloc(reinterpret_cast<void **>(&c->h_si_state), 4 * sizeof(int)) != hipSuccess)
     return cleanup(rc ? rc : fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
   if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 8)) ||
-      (rc = dev_alloc(&c->d_err, 1)) || (rc = dev_alloc(&c->d_ticket, 1)))
+      (rc = dev_alloc(&c->d_err, 1)) ||
+      (rc = dev_alloc(&c->d_ticket, (size_t)kTicketQueues * kTicketStride)))
     return cleanup(rc);
   if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess ||
-      hipMemset(c->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->d_ticket, 0, sizeof(unsigned long long) * kTicketQueues * kTicketStride) !=
+          hipSuccess ||
       hipMemset(c->d_res, 0, 8 * sizeof(double)) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipMemset failed"));
   if (hipHostMalloc(reinterpret_cast<void **>(&c->h_res), 8 * sizeof(double)) != hipSuccess)

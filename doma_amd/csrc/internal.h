@@ -24,6 +24,11 @@ int hip_fail(hipError_t e, const char * what);
       return ::doma::hip_fail(_e, #call);                                                        \
   } while (0)
 
+// Ticket queues of the dataflow sweep: at most kTicketQueues counters, each on
+// its own 256-byte line (kTicketStride counters apart).
+constexpr int kTicketQueues = 16;
+constexpr int kTicketStride = 32;
+
 // Arguments of the brick-wavefront sweep kernel (2D and 3D).
 struct BrickArgs
 {
@@ -58,14 +63,19 @@ struct BrickArgs
   int ntasks;                       // nbricks * ndirsets, ticket order = diagonal order
   uint32_t * flags;                 // [ndirsets][nbricks] epoch of the last finished brick
   uint32_t epoch;                   // this sweep's epoch (> every flag value before it)
-  unsigned long long * ticket;      // monotone ticket counter (never reset)
-  unsigned long long ticket_base;   // counter value at the start of this sweep
+  // Tickets come from nqueues monotone counters, kTicketStride apart: work-group
+  // b takes the tasks of queue b % nqueues, whose direction-set pairs are those
+  // r with r % nqueues == queue.  nqueues divides the pairs per brick and the
+  // grid, so every queue holds ntasks / nqueues tasks and grid / nqueues groups.
+  unsigned long long * ticket;      // [nqueues * kTicketStride], never reset mid-run
+  unsigned long long ticket_base;   // sum of every queue's counter at the sweep's start
+  int nqueues;
   int * error;                      // set when a bounded wait expires
   long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
 
   // Device-side source iteration: when *skip != 0 (the iteration has converged)
   // the launch does nothing (null: always run).  A skipped dataflow launch still
-  // advances the ticket counter by ntasks + grid, as a full one does.
+  // advances the ticket counters by ntasks + grid in total, as a full one does.
   const int * skip;
 };
 
@@ -343,6 +353,7 @@ struct doma_ctx
   int ntx = 0, nty = 0, ntz = 0;
   int ntasks = 0;
   int grid_flow = 0;
+  int nqueues = 1;  // ticket queues of the dataflow schedule
   uint32_t * d_flags = nullptr;
   unsigned long long * d_ticket = nullptr;
   int * d_err = nullptr;

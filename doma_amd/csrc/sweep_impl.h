@@ -28,13 +28,16 @@
 // Two schedules share the brick body:
 //   * diagonal: one launch per brick diagonal bi+bj+bk = s;
 //   * dataflow: one persistent launch per sweep.  Work-groups take (brick,
-//     octant group, chunk) tasks from a monotone ticket counter in diagonal
-//     order and wait on the epoch flags of the (at most three) upwind tasks.  A
-//     task only waits on smaller tickets, taken by work-groups that are already
-//     running, so the schedule cannot deadlock; every wait is bounded by a
-//     wall-clock deadline and reports through an error flag.  Face planes are
-//     written and read with device-coherent accesses, so no cache-wide
-//     write-back or invalidate is needed per task.
+//     octant group, chunk) tasks from monotone ticket counters in diagonal
+//     order and wait on the epoch flags of the (at most three) upwind tasks.
+//     The direction-set pairs are split over up to 16 counters (queues), each
+//     served by its own share of the work-groups, so that no single counter
+//     serialises the whole launch's atomics.  A task only waits on smaller
+//     tickets of its own queue, taken by work-groups that are already running,
+//     so the schedule cannot deadlock; every wait is bounded by a wall-clock
+//     deadline and reports through an error flag.  Face planes are written and
+//     read with device-coherent accesses, so no cache-wide write-back or
+//     invalidate is needed per task.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -87,6 +90,9 @@ constexpr Shape kShapes[] = {
     {3, 8, 8, 1, 2, 512},   // 26: 8-direction sets, 2 octants share a pass (32 cells each)
     {3, 8, 4, 1, 4, 512},   // 27: 4-direction sets, 4 octants share a pass (32 cells each)
     {3, 4, 4, 1, 8, 256},   // 28: 4-direction sets, 4^3 bricks, 8 octants share a pass
+    {3, 4, 4, 1, 1, 64},    // 29: 4-direction sets, 4^3 bricks, one wave
+    {3, 4, 4, 1, 2, 128},   // 30: 4-direction sets, 4^3 bricks, 2 octants
+    {3, 4, 8, 1, 1, 128},   // 31: 8-direction sets, 4^3 bricks, 16 cells per pass
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
@@ -626,9 +632,10 @@ brick_flow_kernel(BrickArgs a)
 {
   if (a.skip && *a.skip)
   {
-    // Keep the ticket counter where a full launch would leave it.
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-      atomicAdd(a.ticket, (unsigned long long)a.ntasks + gridDim.x);
+    // Keep the ticket counters where a full launch would leave them.
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.nqueues)
+      atomicAdd(a.ticket + (size_t)threadIdx.x * kTicketStride,
+                ((unsigned long long)a.ntasks + gridDim.x) / a.nqueues);
     return;
   }
   constexpr Shape S = kShapes[SHAPE];
@@ -638,6 +645,15 @@ brick_flow_kernel(BrickArgs a)
   load_plane_table<R, DIM, S.B, S.DL, S.NDL, S.KO, S.NT>(a, smem);
   const int nbricks = a.ntx * a.nty * a.ntz;
   const int per_brick = a.ndirsets / S.KO;  // (octant group, chunk) pairs
+  // This work-group's ticket queue: pairs r = queue, queue + nq, ... of every
+  // brick, in brick (= diagonal) order.  Upwind tasks share the pair and so the
+  // queue, and hold smaller tickets in it.
+  const int nq = a.nqueues;
+  const int queue = (int)(blockIdx.x % nq);
+  const int pq = per_brick / nq;  // pairs per brick in one queue
+  unsigned long long * ticket = a.ticket + (size_t)queue * kTicketStride;
+  const unsigned long long base = a.ticket_base / nq;
+  const unsigned long long ntq = (unsigned long long)(a.ntasks / nq);
   while (true)
   {
     // A barrier heads every iteration so that the divergent "publish flag" of
@@ -646,16 +662,16 @@ brick_flow_kernel(BrickArgs a)
     __syncthreads();
     if (threadIdx.x == 0)
     {
-      const unsigned long long t = atomicAdd(a.ticket, 1ull) - a.ticket_base;
-      *s_task = t < (unsigned long long)a.ntasks ? (int)t : -1;
+      const unsigned long long u = atomicAdd(ticket, 1ull) - base;
+      *s_task = u < ntq ? (int)u : -1;
     }
     __syncthreads();
     // Uniform by construction; readfirstlane lets the compiler see it.
-    const int t = __builtin_amdgcn_readfirstlane(*s_task);
-    if (t < 0)
+    const int u = __builtin_amdgcn_readfirstlane(*s_task);
+    if (u < 0)
       break;
-    const int br = t / per_brick;
-    const int r = t - br * per_brick;
+    const int br = u / pq;
+    const int r = (u - br * pq) * nq + queue;
     const int og = r / a.nchunks, ch = r % a.nchunks;
     const int bi = a.bricks[3 * br + 0], bj = a.bricks[3 * br + 1], bk = a.bricks[3 * br + 2];
     uint32_t * flags = a.flags + (size_t)r * nbricks;

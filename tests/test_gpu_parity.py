@@ -97,6 +97,9 @@ SWEEP_SHAPES = [
     (28, 3, 17, 16, 19, 4, 4, capi.SCHEME_DD),
     (28, 3, 10, 9, 7, 2, 2, capi.SCHEME_TW_DD),
     (16, 3, 12, 10, 9, 8, 4, capi.SCHEME_TW_DD),
+    (29, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
+    (30, 3, 13, 9, 11, 8, 4, capi.SCHEME_DD),
+    (31, 3, 10, 9, 7, 4, 4, capi.SCHEME_TW_DD),
 ]
 
 
