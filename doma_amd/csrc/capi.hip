@@ -148,6 +148,8 @@ free_sweep(doma_ctx * c)
   dev_free(c->d_fz);
   dev_free(c->d_part);
   dev_free(c->d_flags);
+  dev_free(c->d_pencils);
+  c->npencils = 0;
 }
 
 // Reports (and clears) an expired dependency wait of the dataflow schedule.
@@ -272,6 +274,10 @@ configure_sweep(doma_ctx * c)
   }
   if (c->dims == 3 && c->tuning.schedule == 2)
     return fail(DOMA_ERR_INVALID, "schedule 2 (one lane per row) is a 2D sweep");
+  if (c->tuning.schedule == 3 && c->dims != 3)
+    return fail(DOMA_ERR_INVALID, "schedule 3 (pencils of bricks along z) is a 3D sweep");
+  if (c->tuning.schedule < 0 || c->tuning.schedule > 3)
+    return fail(DOMA_ERR_INVALID, "unknown sweep schedule " + std::to_string(c->tuning.schedule));
   const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
   const int shape = select_shape(c->dims, c->noct, c->dirs, minext, c->tuning);
   ShapeInfo si{};
@@ -288,7 +294,7 @@ configure_sweep(doma_ctx * c)
   c->ko = si.ko;
   c->dw = si.dl * si.ndl;
   c->exact = c->tuning.exact_arith ? 1 : 0;
-  c->schedule = c->tuning.schedule == 1 ? 1 : 0;
+  c->schedule = c->tuning.schedule == 1 || c->tuning.schedule == 3 ? c->tuning.schedule : 0;
   c->ndirsets = c->noct * c->nchunks;
 
   const int ntx = (c->nx + B - 1) / B, nty = (c->ny + B - 1) / B;
@@ -320,7 +326,24 @@ configure_sweep(doma_ctx * c)
   c->nty = nty;
   c->ntz = ntz;
   c->ntasks = (int)(bricks.size() / 3) * (c->ndirsets / c->ko);
-  c->launches_per_sweep = c->schedule == 1 ? 1 : ndiag;
+  c->launches_per_sweep = c->schedule == 0 ? ndiag : 1;
+
+  // Pencils (bi, bj) in diagonal order bi+bj: the upwind pencils of a pencil
+  // come before it, as the upwind bricks of a brick do.
+  std::vector<int16_t> pencils;
+  if (c->schedule == 3)
+  {
+    for (int s = 0; s < ntx + nty - 1; ++s)
+      for (int bj = 0; bj < nty; ++bj)
+      {
+        const int bi = s - bj;
+        if (bi < 0 || bi >= ntx)
+          continue;
+        pencils.push_back((int16_t)bi);
+        pencils.push_back((int16_t)bj);
+      }
+    c->npencils = (int)(pencils.size() / 2);
+  }
 
   // Brick-local cells ordered by hyperplane li+lj+lk.
   const int bz = c->dims == 3 ? B : 1;
@@ -352,6 +375,13 @@ configure_sweep(doma_ctx * c)
                      hipMemcpyHostToDevice));
   DOMA_HIP(hipMemcpy(c->d_plane_off, off.data(), off.size() * sizeof(int32_t),
                      hipMemcpyHostToDevice));
+  if (c->schedule == 3)
+  {
+    if ((rc = dev_alloc(&c->d_pencils, pencils.size())))
+      return rc;
+    DOMA_HIP(hipMemcpy(c->d_pencils, pencils.data(), pencils.size() * sizeof(int16_t),
+                       hipMemcpyHostToDevice));
+  }
 
   const size_t rs = c->precision == DOMA_FP32_SWEEP ? sizeof(float) : sizeof(double);
   size_t nfx, nfy, nfz;
@@ -381,10 +411,11 @@ configure_sweep(doma_ctx * c)
   // The ticket queues restart with the new task list.
   DOMA_HIP(hipMemset(c->d_ticket, 0, sizeof(unsigned long long) * kTicketQueues * kTicketStride));
   c->ticket_base = 0;
-  if (c->schedule == 1)
+  if (c->schedule == 1 || c->schedule == 3)
   {
     int per_cu = 0;
-    if ((rc = sweep_blocks_per_cu(c->scheme, c->precision, c->exact, c->shape, 1, &per_cu)))
+    if ((rc = sweep_blocks_per_cu(c->scheme, c->precision, c->exact, c->shape, c->schedule,
+                                  &per_cu)))
       return rc;
     hipDeviceProp_t prop;
     DOMA_HIP(hipGetDeviceProperties(&prop, c->device));
@@ -399,12 +430,13 @@ configure_sweep(doma_ctx * c)
         break;
       }
     const int resident = std::max(per_cu, 1) * prop.multiProcessorCount;
-    int grid = std::min(c->ntasks, resident);
+    const int tasks = c->schedule == 3 ? c->npencils * pairs : c->ntasks;
+    int grid = std::min(tasks, resident);
     grid = std::max(c->nqueues, grid / c->nqueues * c->nqueues);
     if (grid > resident)
     {
       c->nqueues = 1;
-      grid = std::max(1, std::min(c->ntasks, resident));
+      grid = std::max(1, std::min(tasks, resident));
     }
     c->grid_flow = grid;
   }
@@ -491,9 +523,12 @@ run_sweep_with(doma_ctx * c, const double * q)
   }
   BrickArgs a = brick_args(c, q);
   a.skip = c->si_device ? c->d_si_state : nullptr;
-  if (c->schedule == 1)
+  if (c->schedule == 1 || c->schedule == 3)
   {
-    a.ntasks = c->ntasks;
+    // Pencil schedule: one task per (pencil, direction-set pair).
+    a.ntasks = c->schedule == 3 ? c->npencils * (c->ndirsets / c->ko) : c->ntasks;
+    a.pencils = c->d_pencils;
+    a.npencils = c->npencils;
     a.flags = c->d_flags;
     a.epoch = ++c->epoch;
     a.ticket = c->d_ticket;
@@ -504,12 +539,12 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.wait_ticks = (long long)c->wall_clock_khz * 250;
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
-    int rc = launch_brick_sweep(a, c->scheme, c->precision, c->exact, c->shape, 1, c->grid_flow,
-                                c->stream);
+    int rc = launch_brick_sweep(a, c->scheme, c->precision, c->exact, c->shape, c->schedule,
+                                c->grid_flow, c->stream);
     if (rc)
       return rc;
     // Every work-group takes one ticket past the last task before it exits.
-    c->ticket_base += (unsigned long long)c->ntasks + (unsigned long long)c->grid_flow;
+    c->ticket_base += (unsigned long long)a.ntasks + (unsigned long long)c->grid_flow;
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[1], c->stream));
     return DOMA_OK;
@@ -1806,7 +1841,7 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
     return rc;
   if (g < 0 || g >= c->G)
     return fail(DOMA_ERR_INVALID, "doma_autotune: bad group");
-  // Every kernel shape valid for this problem, under both schedules.
+  // Every kernel shape valid for this problem, under each brick schedule.
   std::vector<doma_tuning> cands;
   if (c->dims == 1)
     cands.push_back(doma_tuning{});
@@ -1814,8 +1849,10 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
   {
     const int minext = c->dims == 3 ? std::min(c->nx, std::min(c->ny, c->nz)) : std::min(c->nx, c->ny);
     for (int s = 0; s < num_shapes(); ++s)
-      for (int sch : {0, 1})
+      for (int sch : {0, 1, 3})
       {
+        if (sch == 3 && c->dims != 3)
+          continue;
         doma_tuning t{};
         t.shape = s + 1;
         t.schedule = sch;

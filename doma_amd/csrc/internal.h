@@ -70,6 +70,10 @@ struct BrickArgs
   unsigned long long * ticket;      // [nqueues * kTicketStride], never reset mid-run
   unsigned long long ticket_base;   // sum of every queue's counter at the sweep's start
   int nqueues;
+  // Pencil schedule: a task is a column of bricks along z, swept bk = 0, 1, ...
+  // by one work-group that keeps the z faces in LDS.
+  const int16_t * pencils;          // [npencils][2] oriented (bi, bj), sorted by bi+bj
+  int npencils;
   int * error;                      // set when a bounded wait expires
   long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
 
@@ -353,6 +357,8 @@ struct doma_ctx
   int ntx = 0, nty = 0, ntz = 0;
   int ntasks = 0;
   int grid_flow = 0;
+  int16_t * d_pencils = nullptr;  // pencil schedule: [npencils][2]
+  int npencils = 0;
   int nqueues = 1;  // ticket queues of the dataflow schedule
   uint32_t * d_flags = nullptr;
   unsigned long long * d_ticket = nullptr;

@@ -246,8 +246,11 @@ struct NoWait
   __device__ void operator()() const {}
 };
 
+// PENCIL: the brick is one of a column along z swept by the same work-group;
+// its z faces never cross global memory (zero inflow at bk = 0, else the
+// previous brick's outflow, still in LDS).
 template <typename R, int DIM, int SCHEME, bool EXACT, int B, int DL, int NDL, int KO, int NT,
-          bool COHERENT, typename Wait = NoWait>
+          bool COHERENT, bool PENCIL = false, typename Wait = NoWait>
 __device__ __forceinline__ void
 sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigned char * smem,
             Wait wait_upwind = Wait{})
@@ -395,8 +398,20 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   wait_upwind();
   __syncthreads();
 
-  constexpr int NF = KO * NAX * FS * DW;   // face values
+  static_assert(!PENCIL || DIM == 3, "pencils are columns of 3D bricks");
+  constexpr int NAXG = PENCIL ? 2 : NAX;   // axes whose faces cross global memory
+  constexpr int NF = KO * NAXG * FS * DW;  // face values loaded
   constexpr int FI = (NF + NT - 1) / NT;   // per lane
+  // LDS index of loaded face value e (the LDS keeps all NAX axes per octant).
+  auto face_slot = [](int e) {
+    return PENCIL ? (e / (NAXG * FS * DW)) * (NAX * FS * DW) + e % (NAXG * FS * DW) : e;
+  };
+  if constexpr (PENCIL)
+  {
+    if (bk == 0)
+      for (int e = tid; e < KO * FS * DW; e += NT)
+        faces[(e / (FS * DW)) * (NAX * FS * DW) + 2 * FS * DW + e % (FS * DW)] = R(0);
+  }
   constexpr int FG = FI < 4 ? FI : 4;      // loads in flight per group
 #pragma unroll
   for (int f0 = 0; f0 < FI; f0 += FG)
@@ -409,8 +424,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
       val[k] = R(0);
       if (f0 + k < FI && e < NF)
       {
-        const int k_o = e / (NAX * FS * DW);
-        const int r = e - k_o * (NAX * FS * DW);
+        const int k_o = e / (NAXG * FS * DW);
+        const int r = e - k_o * (NAXG * FS * DW);
         const int ax = r / (FS * DW);
         const int s = (r / DW) % FS, l = r % DW;
         const int u = DIM == 3 ? s / B : s;
@@ -434,7 +449,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
     {
       const int e = tid + (f0 + k) * NT;
       if (f0 + k < FI && e < NF)
-        faces[e] = val[k];
+        faces[face_slot(e)] = val[k];
     }
   }
   store_cells(0, cq, csg, clx, cly, clz, ccell);
@@ -551,10 +566,10 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   }
 
   // Outflow faces for the downwind bricks (padded extents keep every index in range).
-  for (int e = tid; e < KO * NAX * FS * DW; e += NT)
+  for (int e = tid; e < NF; e += NT)
   {
-    const int k_o = e / (NAX * FS * DW);
-    const int r = e - k_o * (NAX * FS * DW);
+    const int k_o = e / (NAXG * FS * DW);
+    const int r = e - k_o * (NAXG * FS * DW);
     const int ax = r / (FS * DW);
     const int s = (r / DW) % FS, l = r % DW;
     const int u = DIM == 3 ? s / B : s;
@@ -569,7 +584,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
                               : ((size_t)dsk * a.nxp + (i0 + u))) * DW + l;
     else
       dst = gf[2] + ((size_t)dsk * a.nxp * a.nyp + (size_t)(i0 + u) * a.nyp + (j0 + v)) * DW + l;
-    face_store<COHERENT>(dst, faces[e]);
+    face_store<COHERENT>(dst, faces[face_slot(e)]);
   }
 }
 
@@ -697,6 +712,78 @@ brick_flow_kernel(BrickArgs a)
   }
 }
 
+// One persistent launch per sweep whose tasks are pencils: (bi, bj, all bk) for
+// one (octant group, chunk) pair.  The work-group sweeps the column's bricks in
+// z order and keeps their z faces in LDS, so only the x and y faces cross
+// global memory and only the x and y neighbours are waited for.  Pencils are
+// ticketed in bi+bj order per queue as bricks are in brick_flow_kernel; a
+// pencil's brick bk waits only on brick bk of the two upwind pencils, which hold
+// smaller tickets of the same queue, so the induction that rules out deadlock
+// there holds brick by brick.
+template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
+__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+brick_pencil_kernel(BrickArgs a)
+{
+  if (a.skip && *a.skip)
+  {
+    if (blockIdx.x == 0 && (int)threadIdx.x < a.nqueues)
+      atomicAdd(a.ticket + (size_t)threadIdx.x * kTicketStride,
+                ((unsigned long long)a.ntasks + gridDim.x) / a.nqueues);
+    return;
+  }
+  constexpr Shape S = kShapes[SHAPE];
+  using L = Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int * s_task = reinterpret_cast<int *>(smem + L::TASK_OFF);
+  load_plane_table<R, DIM, S.B, S.DL, S.NDL, S.KO, S.NT>(a, smem);
+  const int nbricks = a.ntx * a.nty * a.ntz;
+  const int per_pencil = a.ndirsets / S.KO;  // (octant group, chunk) pairs
+  const int nq = a.nqueues;
+  const int queue = (int)(blockIdx.x % nq);
+  const int pq = per_pencil / nq;
+  unsigned long long * ticket = a.ticket + (size_t)queue * kTicketStride;
+  const unsigned long long base = a.ticket_base / nq;
+  const unsigned long long ntq = (unsigned long long)(a.ntasks / nq);
+  while (true)
+  {
+    __syncthreads();  // see brick_flow_kernel
+    if (threadIdx.x == 0)
+    {
+      const unsigned long long u = atomicAdd(ticket, 1ull) - base;
+      *s_task = u < ntq ? (int)u : -1;
+    }
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(*s_task);
+    if (u < 0)
+      break;
+    const int pc = u / pq;
+    const int r = (u - pc * pq) * nq + queue;
+    const int og = r / a.nchunks, ch = r % a.nchunks;
+    const int bi = a.pencils[2 * pc + 0], bj = a.pencils[2 * pc + 1];
+    uint32_t * flags = a.flags + (size_t)r * nbricks;
+    for (int bk = 0; bk < a.ntz; ++bk)
+    {
+      auto wait_upwind = [&]() {
+        if (threadIdx.x == 0)
+        {
+          if (bi > 0)
+            wait_flag(a, flags + (bk * a.nty + bj) * a.ntx + (bi - 1));
+          if (bj > 0)
+            wait_flag(a, flags + (bk * a.nty + (bj - 1)) * a.ntx + bi);
+        }
+      };
+      sweep_brick<R, DIM, SCHEME, EXACT, S.B, S.DL, S.NDL, S.KO, S.NT, true, true>(
+          a, bi, bj, bk, og, ch, smem, wait_upwind);
+      // Every thread's coherent face stores must have completed before the flag.
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // Batched persistent launch: the tasks of many 3D problems (same kernel shape
 // and quadrature, any extents) in one ticket order, built on the host so that
 // every task's upwind tasks hold smaller tickets (per problem, diagonal order;
@@ -803,7 +890,7 @@ run_shape(const BrickArgs & a, int op, int schedule, int grid, hipStream_t s, in
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
   }
-  else
+  else if (schedule == 1)
   {
     auto k = brick_flow_kernel<R, DIM, SCHEME, EXACT, SHAPE>;
     static bool ready = false;
@@ -817,6 +904,22 @@ run_shape(const BrickArgs & a, int op, int schedule, int grid, hipStream_t s, in
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
   }
+  else if constexpr (DIM == 3)
+  {
+    auto k = brick_pencil_kernel<R, DIM, SCHEME, EXACT, SHAPE>;
+    static bool ready = false;
+    int rc = prepare(k, shmem, ready);
+    if (rc)
+      return rc;
+    if (op == 1)
+    {
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
+      return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
+  }
+  else
+    return fail(DOMA_ERR_INVALID, "the pencil schedule needs a 3D kernel shape");
   return DOMA_OK;
 }
 

@@ -87,7 +87,9 @@ typedef struct doma_tuning {
   int32_t schedule;       /* 0: one launch per brick diagonal (default in 3D);
                              1: one persistent dataflow launch per sweep;
                              2: 2D only, one lane per mesh row, one work-group per
-                                direction (default in 2D up to 512 rows) */
+                                direction (default in 2D up to 512 rows);
+                             3: 3D only, persistent dataflow launch whose tasks
+                                are pencils of bricks along z */
   int32_t octants;        /* octants (2D: quadrants) per work-group: 1, 2, 4 or 8 */
   int32_t dirs_per_lane;  /* directions per lane: 1 or 2 */
   int32_t shape;          /* explicit kernel shape id + 1 (overrides the hints) */

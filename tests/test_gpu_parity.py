@@ -103,12 +103,14 @@ SWEEP_SHAPES = [
 ]
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 3])
 @pytest.mark.parametrize("exact", [0, 1])
 @pytest.mark.parametrize("case", SWEEP_SHAPES, ids=lambda s: f"shape{s[0]}_{s[1]}d_{s[2]}x{s[3]}x"
                          f"{s[4]}_{s[5]}x{s[6]}_s{s[7]}")
 def test_single_sweep_matches_oracle(case, exact, schedule):
     shape_id, dims, nx, ny, nz, np_, na, scheme = case
+    if schedule == 3 and dims != 3:
+        pytest.skip("the pencil schedule is 3D only")
     tuning = dict(shape=shape_id + 1)
     shape = case
     p = _random_problem(dims, nx, ny, nz, 2, np_, na, scheme, seed=nx * 1000 + ny)
@@ -141,27 +143,31 @@ def test_slab_five_region_1d():
     assert np.all(rel_err(phi, ref) <= FP64_TOL)
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 3])
 def test_deterministic_bitwise(schedule):
     """Fixed reduction order: repeated solves are bitwise identical, and the
-    dataflow and diagonal schedules agree bitwise (same per-cell arithmetic)."""
+    diagonal, dataflow and pencil schedules agree bitwise (same per-cell
+    arithmetic)."""
     p, z, _ = load_golden("random_blocks_16c_3g_128d")
     a, _ = _solve(p, z, schedule=schedule)
     b, _ = _solve(p, z, schedule=schedule)
     np.testing.assert_array_equal(a, b)
-    c, _ = _solve(p, z, schedule=1 - schedule)
+    c, _ = _solve(p, z, schedule=0 if schedule else 1)
     np.testing.assert_array_equal(a, c)
 
 
+@pytest.mark.parametrize("schedule", [1, 3])
 @pytest.mark.parametrize("name", ["random_blocks_64c_3g_128d", "kobayashi1_20c_800d_twdd",
                                   "twdd2d_small", "random_blocks_24c_3g_32d_dd"])
-def test_dataflow_schedule_matches_reference(name):
+def test_dataflow_schedule_matches_reference(name, schedule):
     p, z, man = load_golden(name)
-    phi, st = _solve(p, z, schedule=1)
+    if schedule == 3 and p.dims != 3:
+        pytest.skip("the pencil schedule is 3D only")
+    phi, st = _solve(p, z, schedule=schedule)
     ref = z["phi"] if "phi" in z.files else None
     err = rel_err(phi, ref) if ref is not None else rel_err(phi[:, z["sample_idx"]],
                                                             z["phi_sample"])
-    print(f"dataflow {name}: max rel err per group {err}", flush=True)
+    print(f"dataflow {name} schedule {schedule}: max rel err per group {err}", flush=True)
     assert st["success"] == bool(z["ref_success"])
     assert np.all(err <= FP64_TOL)
 

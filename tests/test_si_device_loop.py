@@ -1,7 +1,7 @@
 """The device-side stopping rule of source iteration (chunks of iterations per
 host round trip, kernels after the last iteration returning at entry) against
 the per-iteration host loop: identical iterates (bitwise), iteration counts and
-residuals, for both sweep schedules, 1D with DSA, eigenvalue and transient
+residuals, for the 3D brick sweep schedules, 1D with DSA, eigenvalue and transient
 solves, and source iterations that hit the iteration limit."""
 from __future__ import annotations
 
@@ -24,7 +24,7 @@ def _both(problem, run, tuning=None, dsa=False):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 3])
 def test_fixed_source_identical(schedule):
     p = workloads.random_blocks(24, 3, 2, 2, block=8)
     print("device loop, schedule", schedule, flush=True)
