@@ -104,10 +104,12 @@ struct doma_batch
   int * d_nblocks = nullptr;      // [P]
   double * d_res = nullptr;       // [P][4]
   double * h_res = nullptr;       // pinned [P][4]
-  unsigned long long * d_ticket = nullptr;
+  unsigned long long * d_ticket = nullptr;  // kTicketQueues counters, kTicketStride apart
   int * d_err = nullptr;
   uint32_t epoch = 0;
-  unsigned long long ticket_base = 0;
+  unsigned long long qbase[kTicketQueues] = {};  // each counter's value before the next launch
+  int qoff[kTicketQueues + 1] = {};              // the uploaded task list's queue offsets
+  int nq = 1;
   int wall_clock_khz = 100000;
 
   int sample_cap = 0;
@@ -194,17 +196,20 @@ configure(doma_batch * b)
   hipDeviceProp_t prop;
   DOMA_HIP(hipGetDeviceProperties(&prop, b->device));
   b->grid = std::max(1, std::max(per_cu, 1) * prop.multiProcessorCount);
+  b->nq = std::max(1, std::min(kTicketQueues, b->grid));
   b->tuning = t;
   b->configured = true;
   return DOMA_OK;
 }
 
 // Ticket order of the active problems: by brick diagonal first, then problem,
-// brick and direction-set pair, so every task's upwind tasks come earlier.
+// brick and direction-set pair, so every task's upwind tasks come earlier; the
+// tasks of each ticket queue ((problem * pairs + pair) % nq) are kept in that
+// order and the queues are stored one after the other.
 int
 upload_tasks(doma_batch * b, const std::vector<int> & active, int * ntasks)
 {
-  std::vector<uint32_t> tasks;
+  std::vector<std::vector<uint32_t>> queues(b->nq);
   int maxdiag = 0;
   for (int p : active)
     maxdiag = std::max(maxdiag, (int)b->ctx[p]->diag_off.size() - 1);
@@ -216,8 +221,16 @@ upload_tasks(doma_batch * b, const std::vector<int> & active, int * ntasks)
         continue;
       for (int br = off[s]; br < off[s + 1]; ++br)
         for (int r = 0; r < b->pairs; ++r)
-          tasks.push_back(((uint32_t)p << 20) | ((uint32_t)br << 6) | (uint32_t)r);
+          queues[(p * b->pairs + r) % b->nq].push_back(((uint32_t)p << 20) |
+                                                       ((uint32_t)br << 6) | (uint32_t)r);
     }
+  std::vector<uint32_t> tasks;
+  for (int q = 0; q < b->nq; ++q)
+  {
+    b->qoff[q] = (int)tasks.size();
+    tasks.insert(tasks.end(), queues[q].begin(), queues[q].end());
+  }
+  b->qoff[b->nq] = (int)tasks.size();
   *ntasks = (int)tasks.size();
   if (!tasks.empty())
     DOMA_HIP(hipMemcpyAsync(b->d_tasks, tasks.data(), tasks.size() * sizeof(uint32_t),
@@ -278,15 +291,22 @@ launch_round(doma_batch * b, int ntasks)
   a.ntasks = ntasks;
   a.epoch = ++b->epoch;
   a.ticket = b->d_ticket;
-  a.ticket_base = b->ticket_base;
+  a.nqueues = b->nq;
+  for (int q = 0; q < b->nq; ++q)
+    a.qbase[q] = b->qbase[q];
+  for (int q = 0; q <= b->nq; ++q)
+    a.qoff[q] = b->qoff[q];
   a.error = b->d_err;
   a.wait_ticks = (long long)b->wall_clock_khz * 250;
-  const int grid = std::max(1, std::min(ntasks, b->grid));
+  // A whole number of work-groups per queue, within the resident grid.
+  const int grid = std::max(b->nq, std::min(ntasks, b->grid) / b->nq * b->nq);
   int rc = launch_brick_batch(a, b->scheme, b->precision, b->shape, grid, b->stream);
   if (rc)
     return rc;
-  // Every work-group takes one ticket past the last task before it exits.
-  b->ticket_base += (unsigned long long)ntasks + (unsigned long long)grid;
+  // Every work-group takes one ticket past the last task of its queue.
+  for (int q = 0; q < b->nq; ++q)
+    b->qbase[q] += (unsigned long long)(b->qoff[q + 1] - b->qoff[q]) +
+                   (unsigned long long)(grid / b->nq);
   // Keep the contexts' own epochs above every flag value the batch has written.
   for (auto * c : b->ctx)
     c->epoch = b->epoch;
@@ -435,9 +455,11 @@ doma_batch_create(int32_t count, const doma_problem_desc * descs, doma_batch ** 
       return cleanup(rc);
   if ((rc = dalloc(&b->d_probs, count)) || (rc = dalloc(&b->d_args, count)) ||
       (rc = dalloc(&b->d_nblocks, count)) || (rc = dalloc(&b->d_res, 4 * (size_t)count)) ||
-      (rc = dalloc(&b->d_ticket, 1)) || (rc = dalloc(&b->d_err, 1)))
+      (rc = dalloc(&b->d_ticket, (size_t)kTicketQueues * kTicketStride)) ||
+      (rc = dalloc(&b->d_err, 1)))
     return cleanup(rc);
-  if (hipMemset(b->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess ||
+  if (hipMemset(b->d_ticket, 0, sizeof(unsigned long long) * kTicketQueues * kTicketStride) !=
+          hipSuccess ||
       hipMemset(b->d_err, 0, sizeof(int)) != hipSuccess)
     return cleanup(fail(DOMA_ERR_DEVICE, "hipMemset failed"));
   if (hipHostMalloc(reinterpret_cast<void **>(&b->h_res), 4 * sizeof(double) * count) !=

@@ -84,17 +84,22 @@ struct BrickArgs
 };
 
 // Arguments of the batched persistent sweep: many 3D problems swept with one
-// kernel shape in one launch, the tasks of all of them in one ticket order.  A
-// task code is (problem << 20) | (brick << 6) | pair, where brick indexes the
-// problem's diagonal-sorted brick list and pair = octant group * nchunks + chunk.
+// kernel shape in one launch.  A task code is (problem << 20) | (brick << 6) |
+// pair, where brick indexes the problem's diagonal-sorted brick list and pair =
+// octant group * nchunks + chunk.  Tasks are split over nqueues ticket queues by
+// (problem * pairs + pair) % nqueues, so a task and its upwind tasks share a
+// queue; queue q's tasks are tasks[qoff[q] .. qoff[q+1]) in ticket order, served
+// by the work-groups b with b % nqueues == q.
 struct BatchArgs
 {
   const BrickArgs * probs;         // [P] per-problem sweep arguments (device)
-  const uint32_t * tasks;          // [ntasks]
+  const uint32_t * tasks;          // [ntasks], grouped by queue
   int ntasks;
   uint32_t epoch;                  // > every flag value of every problem before it
-  unsigned long long * ticket;     // monotone ticket counter (never reset)
-  unsigned long long ticket_base;  // counter value at the start of this launch
+  unsigned long long * ticket;     // [kTicketQueues * kTicketStride] monotone counters
+  unsigned long long qbase[kTicketQueues];  // each queue's counter at the launch's start
+  int qoff[kTicketQueues + 1];
+  int nqueues;
   int * error;                     // set when a bounded wait expires
   long long wait_ticks;            // per-dependency deadline in wall-clock ticks
 };

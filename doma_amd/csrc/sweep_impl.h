@@ -800,13 +800,20 @@ brick_batch_kernel(BatchArgs b)
   int * s_task = reinterpret_cast<int *>(smem + L::TASK_OFF);
   // Every problem of a batch shares the brick edge and so the plane table.
   load_plane_table<R, 3, S.B, S.DL, S.NDL, S.KO, S.NT>(b.probs[0], smem);
+  // This work-group's ticket queue (see brick_flow_kernel): its tasks are
+  // tasks[qoff[queue] ..) and their upwind tasks sit earlier in the same queue.
+  const int queue = (int)(blockIdx.x % b.nqueues);
+  unsigned long long * ticket = b.ticket + (size_t)queue * kTicketStride;
+  const unsigned long long base = b.qbase[queue];
+  const int first = b.qoff[queue];
+  const unsigned long long count = (unsigned long long)(b.qoff[queue + 1] - first);
   while (true)
   {
     __syncthreads();  // see brick_flow_kernel
     if (threadIdx.x == 0)
     {
-      const unsigned long long t = atomicAdd(b.ticket, 1ull) - b.ticket_base;
-      *s_task = t < (unsigned long long)b.ntasks ? (int)t : -1;
+      const unsigned long long u = atomicAdd(ticket, 1ull) - base;
+      *s_task = u < count ? first + (int)u : -1;
     }
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(*s_task);
