@@ -24,6 +24,29 @@ namespace
 {
 constexpr int kAhead = 8;  // steps between a cell's loads and its solve
 
+// Lane i receives lane i - 1's value (lane 0 keeps its own): the DPP wave shift
+// right by one (wave_shr:1, GFX9 DPP control 0x138), a VALU move instead of an
+// LDS permute on the step-to-step dependency chain.
+__device__ __forceinline__ int
+wave_shr1(int v)
+{
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ double
+wave_shr1(double v)
+{
+  const int lo = wave_shr1(__double2loint(v));
+  const int hi = wave_shr1(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float
+wave_shr1(float v)
+{
+  return __int_as_float(wave_shr1(__float_as_int(v)));
+}
+
 template <typename R, int SCHEME, bool EXACT>
 __device__ __forceinline__ R
 cell2d(double q, double sg, double lx, double ly, R amu, R aeta, R axi, R & x, R & y)
@@ -118,7 +141,7 @@ sweep2d_rows_kernel(Rows2DArgs a)
       // y face from the lane below, produced at step s - 1; lane 0 takes the
       // wave below's top-row face of the same column, once it is ready (the
       // wave below never waits on this one, so the wait ends).
-      R yin = __shfl_up(yo, 1, 64);
+      R yin = wave_shr1(yo);
       const int ii0 = s - 64 * wave;  // lane 0's column (uniform)
       if (wave > 0 && ii0 >= 0 && ii0 < a.nx)
       {
