@@ -1,9 +1,15 @@
 import json
+import os
 import sys
 from pathlib import Path
 
 import numpy as np
 import pytest
+
+# The oracle's OpenMP threads sleep rather than spin while they wait: several
+# test processes sharing the host's cores (pytest -n) otherwise spin against
+# each other.  Read when the oracle library first loads, so it is set here.
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
 
 ROOT = Path(__file__).resolve().parents[1]
 GOLDEN = ROOT / "tests" / "golden"
