@@ -55,11 +55,21 @@ def cpu_baseline(problem, seconds: float, threads: int):
             break
     per = el / n
     units = problem.N * problem.noct * problem.dirs
+    # The same sweep on one thread (two sweeps: about as long as the sample above).
+    n1, t1 = 0, time.perf_counter()
+    while n1 < 2:
+        ob.sweep(problem, 0, q, nthreads=1)
+        n1 += 1
+    per1 = (time.perf_counter() - t1) / n1
     return {"value": per * 1e9 / units, "unit": "ns/(cell*dir*grp)", "cores": threads,
             "kind": "port",
             "sample": f"{n} full sweeps of group 0 of the same 64^3 x 128-direction problem "
                       f"({per * 1e3:.1f} ms/sweep, {el:.1f} s), OpenMP parallel-in-angle "
-                      f"with per-thread partials, {threads} threads"}
+                      f"with per-thread partials, {threads} threads",
+            "host_logical_cpus": os.cpu_count(),
+            "single_thread": {"value": per1 * 1e9 / units, "unit": "ns/(cell*dir*grp)",
+                              "sample": f"{n1} full sweeps of group 0 on 1 thread "
+                                        f"({per1 * 1e3:.0f} ms/sweep)"}}
 
 
 def sweeps_only(args):
