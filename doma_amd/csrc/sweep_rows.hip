@@ -119,7 +119,10 @@ sweep2d_rows_kernel(Rows2DArgs a)
   __syncthreads();
   const R * y_below = s_y + (size_t)(wave - 1) * a.nx;  // wave > 0 only
   R * y_above = s_y + (size_t)wave * a.nx;             // wave < nwaves - 1 only
-  int spins = 0, published = 0;
+  // seen: columns of the wave below known to be ready (uniform).  The wave below
+  // publishes its count once per group of steps, so the count is read only when
+  // this wave runs past the last value it saw.
+  int spins = 0, published = 0, seen = 0;
   R x = R(0), yo = R(0);
   // Eight steps on the inputs in (uq, us) while the next eight cells' inputs
   // load into (lq, ls); two register sets alternate, so no copy waits on them.
@@ -145,17 +148,22 @@ sweep2d_rows_kernel(Rows2DArgs a)
       const int ii0 = s - 64 * wave;  // lane 0's column (uniform)
       if (wave > 0 && ii0 >= 0 && ii0 < a.nx)
       {
-        while (__hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP) <= ii0)
+        if (ii0 >= seen)
         {
-          if (++spins > (1 << 26))
+          while ((seen = __hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP)) <= ii0)
           {
-            if (lane == 0)
-              __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
+            if (++spins > (1 << 26))
+            {
+              if (lane == 0)
+                __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              seen = a.nx;  // give up: the error flag reports the stall
+              break;
+            }
           }
+          // Orders every later read of the column after the count that covers it.
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");  // LDS only
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");  // LDS only
         const R yb = y_below[ii0];
         if (lane == 0)
           yin = yb;
