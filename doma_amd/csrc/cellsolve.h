@@ -115,9 +115,11 @@ __device__ __forceinline__ void
 inv_weights3(double na, double da, bool va, double nb, double db, bool vb, double nc, double dc,
              bool vc, double & ia, double & ib, double & ic)
 {
-  const bool ua = va && na >= 0.0 && na < 0.5 * da;
-  const bool ub = vb && nb >= 0.0 && nb < 0.5 * db;
-  const bool uc = vc && nc >= 0.0 && nc < 0.5 * dc;
+  // na < 0 is tested once, as the negation of na >= 0 (all operands are finite).
+  const bool pa = na >= 0.0, pb = nb >= 0.0, pc = nc >= 0.0;
+  const bool ua = va && pa && na < 0.5 * da;
+  const bool ub = vb && pb && nb < 0.5 * db;
+  const bool uc = vc && pc && nc < 0.5 * dc;
   const double Da = ua ? da - na : 1.0;
   const double Db = ub ? db - nb : 1.0;
   const double Dc = uc ? dc - nc : 1.0;
@@ -125,9 +127,9 @@ inv_weights3(double na, double da, bool va, double nb, double db, bool vb, doubl
   const double r = fast_rcp(P);
   const double rDa = r * Da;
   const double ra = r * Dbc, rb = rDa * Dc, rc = rDa * Db;
-  ia = ua ? da * ra : (va && na < 0.0 ? 1.0 : 2.0);
-  ib = ub ? db * rb : (vb && nb < 0.0 ? 1.0 : 2.0);
-  ic = uc ? dc * rc : (vc && nc < 0.0 ? 1.0 : 2.0);
+  ia = ua ? da * ra : (va && !pa ? 1.0 : 2.0);
+  ib = ub ? db * rb : (vb && !pb ? 1.0 : 2.0);
+  ic = uc ? dc * rc : (vc && !pc ? 1.0 : 2.0);
 }
 
 // Straight-line TWDD3D: the three weights are always formed and the reference's
