@@ -25,7 +25,7 @@
 //     are summed in a fixed order afterwards, so results are reproducible run to
 //     run (no floating-point atomics).
 //
-// Two schedules share the brick body:
+// Three schedules share the brick body:
 //   * diagonal: one launch per brick diagonal bi+bj+bk = s;
 //   * dataflow: one persistent launch per sweep.  Work-groups take (brick,
 //     octant group, chunk) tasks from monotone ticket counters in diagonal
@@ -37,7 +37,9 @@
 //     so the schedule cannot deadlock; every wait is bounded by a wall-clock
 //     deadline and reports through an error flag.  Face planes are written and
 //     read with device-coherent accesses, so no cache-wide write-back or
-//     invalidate is needed per task.
+//     invalidate is needed per task;
+//   * pencil (3D): the dataflow launch with a task per column of bricks along
+//     z, swept by one work-group that keeps the z faces in LDS.
 #pragma once
 
 #include <hip/hip_runtime.h>
