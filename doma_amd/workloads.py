@@ -65,6 +65,44 @@ def slab_five_region(np_: int = 8, scale: int = 1) -> Problem:
                    mu, eta, xi, w, name=f"slab_five_region_{len(lx)}_S{2 * np_}")
 
 
+def kobayashi1(refinement: int = 0, np_: int = 10, na: int = 10,
+               scheme: int = capi.SCHEME_DD) -> Problem:
+    """The reference's cases/3D_kobayashi_1.xml: a 200 cm cube of interval widths
+    50/40/20/40/50 cm with 5/4/2/4/5 cells each, times (refinement + 1)
+    (InputParameters.cpp ix * (refinement + 1)); block 0 (the central 20 cm cube)
+    is the unit source with sigma_t 0.1, sigma_s 0.05; block 1 (the surrounding
+    60 cm shell) is a near-void duct region (1e-4 / 5e-5); block 2 is the shield
+    (0.1 / 0.05); vacuum boundaries, one group, GC np x na per octant."""
+    widths = [50.0, 40.0, 20.0, 40.0, 50.0]
+    ivals = [k * (refinement + 1) for k in (5, 4, 2, 4, 5)]
+    l = np.concatenate([np.full(n, w / n) for w, n in zip(widths, ivals)])
+    blocks = np.full((5, 5, 5), 2, dtype=np.int32)
+    blocks[1:4, 1:4, 1:4] = 1
+    blocks[2, 2, 2] = 0
+    idx = np.repeat(np.arange(5), ivals)
+    mat = blocks[idx][:, idx][:, :, idx].astype(np.int32)  # [k][j][i]
+    sigt = np.array([[1e-1], [1e-4], [1e-1]])
+    sigs = np.array([5e-2, 5e-5, 5e-2]).reshape(3, 1, 1)
+    src = np.array([[1.0], [0.0], [0.0]])
+    has = np.ones(3, dtype=np.uint8)
+    mu, eta, xi, w = quadrature_3d(np_, na)
+    return Problem(3, l, l.copy(), l.copy(), 1, scheme, mat.reshape(-1), sigt, sigs, src, has,
+                   mu, eta, xi, w, name=f"kobayashi1_{len(l)}c_{8 * np_ * na}d")
+
+
+def cell_at_point(widths, x):
+    """Index of the first cell whose closed interval [x_c - l/2, x_c + l/2] holds x,
+    with the reference's centroids x_c = l_0/2 + sum_(j<i) l_j accumulated in
+    order (BrickMesh3D.cpp:83-101, CartesianCell3D.cpp:60-67, fluxAtPoint
+    BrickMesh3D.cpp:202-213 returns the first such cell in cell order)."""
+    xc = 0.5 * widths[0]
+    for i, li in enumerate(widths):
+        if xc - 0.5 * li <= x <= xc + 0.5 * li:
+            return i
+        xc += li
+    raise ValueError(f"point {x} lies outside the mesh")
+
+
 def _media(spec: MediaSpec):
     return capi.media_generate(spec)
 
