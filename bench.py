@@ -36,6 +36,58 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Host CPU model string and logical CPU count."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline_reference(problem, np_, na, tol, threads, gpu_phi=None, gpu_ms=None):
+    """The reference's own OpenMP solver (TransportSolver3D compiled from its
+    sources, oracle/_ref/ref_harness) solving the same problem end to end:
+    solveFixedSource to the same tolerances, timed inside the harness (mesh
+    construction excluded).  Returns None when the harness was not built."""
+    import re
+    import tempfile
+
+    import numpy as np
+
+    from oracle import binding as ob
+
+    if not ob.REF_HARNESS.exists():
+        return None
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    with tempfile.TemporaryDirectory() as tmp:
+        ok, phi, logtxt = ob.run_reference(problem, np_, na, tol, 1000, tol, 100, tmp,
+                                           nthreads=threads, tag="cpu_baseline")
+    solve_s = float(re.findall(r"solve_seconds (\S+)", logtxt)[-1])
+    sweeps = len(re.findall(r"Performing SI", logtxt))
+    units = problem.N * problem.noct * problem.dirs
+    model, ncpu = host_cpu()
+    out = {"value": round(solve_s * 1e9 / (sweeps * units), 4), "unit": "ns/(cell*dir*grp)",
+           "cores": threads, "kind": "reference",
+           "sample": f"the reference's own solveFixedSource (TransportSolver3D<TWDiamondDifference3D>"
+                     f" compiled from its sources with -O2 -fopenmp) on the same "
+                     f"{problem.name} problem to {tol:g}: {sweeps} sweeps in {solve_s:.1f} s "
+                     f"on {threads} OpenMP threads (parallel in angle)",
+           "time_to_solution_s": round(solve_s, 3), "converged": bool(ok),
+           "host_cpu": model, "host_logical_cpus": ncpu}
+    if gpu_phi is not None:
+        out["gpu_vs_reference_max_rel"] = [
+            float(x) for x in np.abs(gpu_phi - phi).max(axis=1) / np.abs(phi).max(axis=1)]
+    if gpu_ms:
+        out["gpu_speedup_time_to_solution"] = round(solve_s * 1e3 / gpu_ms, 1)
+    return out
+
+
 def cpu_baseline(problem, seconds: float, threads: int):
     """CPU restatement of the reference's OpenMP parallel-in-angle sweep, timed on
     a bounded sample of the same workload (whole sweeps of group 0)."""
@@ -138,88 +190,92 @@ def batch_mode(args):
     print(json.dumps(out), flush=True)
 
 
-def learned_compare(workload="high_scatter", n=64, groups=8, dd=True, every=5, reference=False,
-                    learned_only=False, weights_path=None):
-    """Learned-corrected FP32-sweep SI against plain FP64 GPU SI on the same
-    high-scattering problem, both to a 1e-6 residual, optionally both against
-    plain FP64 SI converged to 1e-9 (no corrections).  With dd the cells use
-    diamond difference, whose source iteration is linear and has one fixed point;
-    the reference's increment-form TW-DD is not additive (its weights depend on
-    each increment, and switch to DD below 1e-8), so its converged flux depends
-    on the iteration path and a corrected path ends at a different one."""
+def learned_accel(n=128, groups=8, every=5, tol=1e-6, ref_tol=1e-10, weights_path=None,
+                  runs=None):
+    """Learned acceleration on the high-scattering workload (n^3 cells of 1 cm,
+    `groups` groups, 256 directions, sigma_t log-uniform in [0.5, 10], c ~
+    U[0.95, 0.995] with 98 % in-group, diamond difference).  Every run is timed
+    end to end (solveFixedSource from zero) and compared with plain FP64 SI
+    converged to `ref_tol`:
+      plain_fp64            the reference's algorithm: cold group restarts, stop
+                            at a `tol` residual;
+      plain_fp64_warm       plain FP64 with warm-started group iterations and the
+                            error-estimate stopping rule (r rho/(1-rho) <= tol);
+      extrapolation_fp32    FP32 sweep, warm start, error-estimate rule, and the
+                            correction with a hand-set network that outputs its
+                            first input: the bare geometric-series extrapolation
+                            g d_k every `every` iterations (the ablation);
+      learned_fp32          the same with the trained network.
+    With diamond difference source iteration is linear, so every run has the
+    same fixed point."""
     import numpy as np
 
-    from doma_amd import workloads
-    from doma_amd.capi import FP32_SWEEP, FP64, SCHEME_DD, Context
+    from doma_amd import unet, workloads
+    from doma_amd.capi import FP32_SWEEP, FP64, STOP_ERROR_ESTIMATE, STOP_RESIDUAL, Context
 
     weights = np.load(weights_path or ROOT / "doma_amd" / "weights" / "si_correction_unet.npy")
-    p = workloads.diffusive(n) if workload == "diffusive" else workloads.high_scatter(n, groups)
-    if dd:
-        p.scheme = SCHEME_DD
-        p.name += "_dd"
-    tol, fs_max, mg_max = 1e-6, 50000, 500
-    log(f"bench: {p.name} N={p.N} groups={p.groups} directions={p.noct * p.dirs}")
-
-    def run(precision, learned, tol_, label, warm=False):
-        ctx = Context(p, precision=precision)
-        ctx.autotune(0)
-        if learned:
-            ctx.set_learned_correction(weights, every=every)
-        if warm:
-            ctx.set_warm_start(True)
+    p = workloads.high_scatter(n, groups)
+    log(f"bench: {p.name} N={p.N} groups={p.groups} directions={p.noct * p.dirs} (DD)")
+    tunings = {}
+    plan = {
+        "reference_fp64": (FP64, None, True, STOP_RESIDUAL, ref_tol),
+        "plain_fp64": (FP64, None, False, STOP_RESIDUAL, tol),
+        "plain_fp64_warm": (FP64, None, True, STOP_ERROR_ESTIMATE, tol),
+        "extrapolation_fp32": (FP32_SWEEP, unet.extrapolation_params(), True, STOP_ERROR_ESTIMATE,
+                               tol),
+        "learned_fp32": (FP32_SWEEP, weights, True, STOP_ERROR_ESTIMATE, tol),
+    }
+    out = {"workload": p.name, "scheme": "dd", "tolerance": tol, "every": every,
+           "reference_tolerance": ref_tol}
+    phis = {}
+    for label in ["reference_fp64"] + [k for k in plan if k != "reference_fp64"
+                                       and (runs is None or k in runs)]:
+        prec, params, warm, rule, t = plan[label]
+        ctx = Context(p, precision=prec)
+        if prec not in tunings:
+            tunings[prec] = ctx.autotune(0)[0]
+        else:
+            ctx.set_tuning(**{k: v for k, v in tunings[prec].items()})
+        if params is not None:
+            ctx.set_learned_correction(params, every=every)
+        ctx.set_warm_start(warm)
+        ctx.set_stopping_rule(rule)
         t0 = time.perf_counter()
-        phi, st = ctx.solve_fixed_source(tol_, fs_max, tol_, mg_max)
+        phi, st = ctx.solve_fixed_source(t, 50000, t, 50)
         wall = time.perf_counter() - t0
         ctx.close()
-        log(f"bench: {label}: {wall:.2f} s, {st['sweeps']} sweeps, MG {st['mg_iterations']}, "
-            f"corrections {st['corrections_accepted']}+/{st['corrections_rejected']}-")
-        return phi, st, wall
-
-    if learned_only:  # profiling: the accelerated solve alone
-        _, st_w, t_w = run(FP32_SWEEP, True, tol, "learned FP32 + warm start", warm=p.groups > 1)
-        return {"workload": p.name, "wall_s": round(t_w, 3), "sweeps": st_w["sweeps"],
-                "mg_iterations": st_w["mg_iterations"],
-                "correction_s": round(st_w["correction_ms"] / 1e3, 3)}
-    phi_p, st_p, t_p = run(FP64, False, tol, "plain FP64")
-    phi_l, st_l, t_l = run(FP32_SWEEP, True, tol, "learned FP32")
-    phi_w, st_w, t_w = (None, None, None)
-    if p.groups > 1:
-        phi_w, st_w, t_w = run(FP32_SWEEP, True, tol, "learned FP32 + warm start", warm=True)
-    out = {"workload": p.name, "tolerance": tol, "every": every,
-           "plain_fp64": {"wall_s": round(t_p, 3), "sweeps": st_p["sweeps"],
-                          "mg_iterations": st_p["mg_iterations"], "success": st_p["success"]},
-           "learned_fp32": {"wall_s": round(t_l, 3), "sweeps": st_l["sweeps"],
-                            "mg_iterations": st_l["mg_iterations"], "success": st_l["success"],
-                            "corrections_accepted": st_l["corrections_accepted"],
-                            "corrections_rejected": st_l["corrections_rejected"],
-                            "correction_s": round(st_l["correction_ms"] / 1e3, 3)},
-           "speedup": round(t_p / t_l, 3)}
-    if st_w is not None:
-        out["learned_fp32_warm"] = {
-            "wall_s": round(t_w, 3), "sweeps": st_w["sweeps"],
-            "mg_iterations": st_w["mg_iterations"], "success": st_w["success"],
-            "corrections_accepted": st_w["corrections_accepted"],
-            "corrections_rejected": st_w["corrections_rejected"]}
-        out["speedup_warm"] = round(t_p / t_w, 3)
-    if reference:
-        phi_r, st_r, t_r = run(FP64, False, 1e-9, "reference: plain FP64 to 1e-9")
-
-        def errs(phi):
-            d = np.abs(phi - phi_r)
-            return {"max_rel": [float(x) for x in d.max(axis=1) / np.abs(phi_r).max(axis=1)],
-                    "l2_rel": [float(x) for x in np.linalg.norm(phi - phi_r, axis=1) /
-                               np.linalg.norm(phi_r, axis=1)]}
-        out["reference"] = {"tolerance": 1e-9, "wall_s": round(t_r, 3),
-                            "sweeps": st_r["sweeps"]}
-        out["plain_fp64"]["error"] = errs(phi_p)
-        out["learned_fp32"]["error"] = errs(phi_l)
-        if st_w is not None:
-            out["learned_fp32_warm"]["error"] = errs(phi_w)
-    else:
-        # the accelerated result against plain SI's (both stopped at 1e-6)
-        ref = phi_w if phi_w is not None else phi_l
-        out["accelerated_vs_plain_max_rel"] = [
-            float(x) for x in np.abs(ref - phi_p).max(axis=1) / np.abs(phi_p).max(axis=1)]
+        phis[label] = phi
+        rec = {"wall_s": round(wall, 3), "sweeps": st["sweeps"],
+               "mg_iterations": st["mg_iterations"], "success": st["success"],
+               "precision": "fp32_sweep" if prec == FP32_SWEEP else "fp64",
+               "warm_start": warm,
+               "stopping_rule": "residual" if rule == STOP_RESIDUAL else "error_estimate"}
+        if params is not None:
+            rec.update(corrections_accepted=st["corrections_accepted"],
+                       corrections_rejected=st["corrections_rejected"],
+                       correction_s=round(st["correction_ms"] / 1e3, 3))
+        if label != "reference_fp64":
+            ref = phis["reference_fp64"]
+            d = phi - ref
+            rec["max_rel_err"] = [float(x) for x in
+                                  np.abs(d).max(axis=1) / np.abs(ref).max(axis=1)]
+            rec["l2_rel_err"] = [float(x) for x in
+                                 np.linalg.norm(d, axis=1) / np.linalg.norm(ref, axis=1)]
+        out[label] = rec
+        log(f"bench: {label}: {wall:.2f} s, {st['sweeps']} sweeps, MG {st['mg_iterations']}"
+            + (f", max err {max(rec['max_rel_err']):.2e}" if "max_rel_err" in rec else "")
+            + (f", corrections {st['corrections_accepted']}+/{st['corrections_rejected']}-"
+               if params is not None else ""))
+    if "learned_fp32" in out:
+        t_l = out["learned_fp32"]["wall_s"]
+        best = min(out[k]["wall_s"] for k in ("plain_fp64", "plain_fp64_warm") if k in out)
+        out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["wall_s"] / t_l, 3) \
+            if "plain_fp64" in out else None
+        out["speedup_vs_best_plain"] = round(best / t_l, 3)
+        if "extrapolation_fp32" in out:
+            out["speedup_vs_extrapolation"] = round(out["extrapolation_fp32"]["wall_s"] / t_l, 3)
+        out["learned_max_err"] = max(out["learned_fp32"]["max_rel_err"])
+    out["tuning"] = {("fp32_sweep" if k == FP32_SWEEP else "fp64"): v for k, v in tunings.items()}
     return out
 
 
@@ -296,10 +352,8 @@ def deck_mode(args):
 
 
 def learned_mode(args):
-    print(json.dumps(learned_compare(args.workload if args.workload in ("diffusive", "high_scatter")
-                                     else "high_scatter", args.n, args.groups, args.dd, args.every,
-                                     not args.no_reference, args.learned_only,
-                                     args.weights)), flush=True)
+    print(json.dumps(learned_accel(args.n, args.groups, args.every, weights_path=args.weights)),
+          flush=True)
 
 
 def copy_calibration(nbytes=256 << 20, reps=3):
@@ -330,10 +384,7 @@ def main():
     ap.add_argument("--deck", default=None, help="input deck (deck mode)")
     ap.add_argument("--autotune", action="store_true", help="deck mode: autotune the sweep")
     ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
-    ap.add_argument("--no-reference", action="store_true")
-    ap.add_argument("--dd", action="store_true", help="diamond difference (learned mode)")
     ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
-    ap.add_argument("--learned-only", action="store_true", help="learned mode: accelerated run only")
     ap.add_argument("--weights", default=None, help="learned mode: correction weights (.npy)")
     ap.add_argument("--count", type=int, default=512, help="problems (batch mode)")
     ap.add_argument("--sample-cap", type=int, default=8, help="snapshots per problem and group")
@@ -405,34 +456,46 @@ def main():
         if (rec.get("tuning_shape") == tuning.get("shape")
                 and rec.get("schedule") == tuning.get("schedule")):
             traffic = rec.get("bytes_per_launch")
-    # The sweep is bound by FP64 arithmetic: MI355X's FP64 vector and FP64 matrix
-    # peaks are the same 78.6 TFLOP/s, so the compute roof is reported under the
-    # "mfma" name even though the cell solves run on the vector ALUs.
-    roofline = {"bound": "mfma", "roof": "fp64 compute (VALU issue; no MFMA in the sweep)",
+    # The sweep is bound by FP64 arithmetic on the vector ALUs (no MFMA: the cell
+    # solve is not a contraction); MI355X's FP64 vector peak is 78.6 TFLOP/s.
+    roofline = {"bound": "valu_fp64", "roof": "fp64 vector compute (no MFMA in the sweep)",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "timing": "HIP events around each sweep launch on the library's stream",
                 "traffic": traffic,
                 "kernel": "brick_flow_kernel" if tuning.get("schedule") == 1 else "brick_sweep_kernel",
                 "avg_launch_us": round(k_ms * 1e3 / launches, 2),
-                "launches_per_sweep": launches, "flops_per_unit": FLOPS_PER_UNIT[p.scheme]}
+                "launches_per_sweep": launches, "flops_per_unit": FLOPS_PER_UNIT[p.scheme],
+                "compulsory_bytes_per_unit": round(24.0 / (p.noct * p.dirs), 4),
+                "hbm_target": "the 50%-of-HBM target is superseded by the FP64 roof: with the "
+                              "upwind faces on chip the compulsory traffic is 24/D B per unit "
+                              "(q, sigma_t, dphi in FP64), so even at the FP64 roof "
+                              "(110 flop/unit) the sweep needs under 2% of 8 TB/s"}
+    # The same kernel timed by rocprofv3 (committed summary), when it covers the
+    # tuned shape: the fraction recomputed from that average duration.
+    pfile = ROOT / "profiles" / "sweep_kernel_timing.json"
+    if pfile.exists():
+        rec = json.loads(pfile.read_text()).get(p.name, {})
+        if (rec.get("tuning_shape") == tuning.get("shape")
+                and rec.get("schedule") == tuning.get("schedule")):
+            ach_p = flops_sweep / (rec["avg_us"] * 1e-6) / 1e12
+            roofline["rocprof_avg_launch_us"] = rec["avg_us"]
+            roofline["frac_rocprof"] = round(ach_p / PEAK_FP64_TFLOPS, 4)
     ctx.close()
     learned = None
     if not args.no_learned:
-        lc = learned_compare("high_scatter", 64, 8, dd=True, every=5)
-        learned = {"workload": lc["workload"], "tolerance": lc["tolerance"],
-                   "plain_fp64_s": lc["plain_fp64"]["wall_s"],
-                   "learned_fp32_s": lc["learned_fp32"]["wall_s"],
-                   "learned_fp32_warm_start_s": lc["learned_fp32_warm"]["wall_s"],
-                   "speedup_learned": lc["speedup"], "speedup_learned_warm_start": lc["speedup_warm"],
-                   "sweeps": {"plain": lc["plain_fp64"]["sweeps"],
-                              "learned": lc["learned_fp32"]["sweeps"],
-                              "learned_warm_start": lc["learned_fp32_warm"]["sweeps"]},
-                   "max_rel_vs_plain": lc["accelerated_vs_plain_max_rel"]}
+        learned = learned_accel(128, 8, 5)
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
-        log(f"bench: CPU baseline with {threads} threads")
-        cpu = cpu_baseline(p, args.cpu_seconds, threads)
+        log(f"bench: CPU baseline: the reference's solver with {threads} threads")
+        with Context(p, precision=FP64, tuning=tuning) as c2:
+            phi_gpu, _ = c2.solve_fixed_source(tol, 1000, tol, 100)
+        cpu = cpu_baseline_reference(p, 4, 4, tol, threads, phi_gpu, ms_per_step)
+        if cpu is None:  # reference harness not built: the CPU restatement
+            cpu = cpu_baseline(p, args.cpu_seconds, threads)
+        else:
+            log(f"bench: reference solve {cpu['time_to_solution_s']} s")
 
     out = {
         "metric": "grind ns/(cell*dir*grp) + time-to-1e-6 residual, learned vs plain SI",

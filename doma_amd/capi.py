@@ -29,6 +29,10 @@ STEP_NONE, STEP_BOTH, STEP_INSERT, STEP_REMOVE = -1, 0, 1, 2
 
 SCATTER_DOWN1 = 0
 SCATTER_FULL = 1
+SCATTER_INGROUP = 2
+STOP_RESIDUAL = 0
+NET_CIN = 4  # input channels of the correction network
+STOP_ERROR_ESTIMATE = 1
 SOURCE_CENTRAL = 0
 SOURCE_BLOCKS = 1
 
@@ -137,6 +141,7 @@ SIGNATURES = [
     ("doma_set_learned_correction", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64,
                                               C.c_int32]),
     ("doma_set_warm_start", C.c_int, [C.c_void_p, C.c_int32]),
+    ("doma_set_stopping_rule", C.c_int, [C.c_void_p, C.c_int32]),
     ("doma_net_forward", C.c_int, [C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32,
                                    C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("doma_net_trainer_create", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_float), C.c_int64,
@@ -161,7 +166,8 @@ SIGNATURES = [
     ("doma_batch_get_flux", C.c_int, [C.c_void_p, C.c_int32, _dp]),
     ("doma_batch_sample_count", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _ip]),
     ("doma_batch_get_sample", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
-                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float)]),
     ("doma_deck_read", C.c_int, [C.c_char_p, C.POINTER(DeckInfo)]),
     ("doma_run_deck", C.c_int,
      [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(Tuning), C.c_int32, _dp, C.c_int64,
@@ -478,6 +484,10 @@ class Context:
         """Start each group's iteration from the previous multigroup iterate."""
         _check(lib().doma_set_warm_start(self.h, int(enable)), "doma_set_warm_start")
 
+    def set_stopping_rule(self, rule: int = STOP_RESIDUAL):
+        """STOP_RESIDUAL (the reference's) or STOP_ERROR_ESTIMATE (r rho / (1 - rho))."""
+        _check(lib().doma_set_stopping_rule(self.h, int(rule)), "doma_set_stopping_rule")
+
     def set_learned_correction(self, params, every: int = 5):
         """Enable (every > 0) or disable (every = 0) the learned SI correction."""
         fp = C.POINTER(C.c_float)
@@ -528,10 +538,12 @@ def net_param_count() -> int:
 
 
 def net_forward(params: np.ndarray, features: np.ndarray) -> np.ndarray:
-    """Stand-alone GPU forward pass: features [D, H, W, 3] (FP32) -> [D, H, W]."""
+    """Stand-alone GPU forward pass: features [D, H, W, 4] (FP32) -> [D, H, W]."""
     params = np.ascontiguousarray(params, dtype=np.float32)
     f = np.ascontiguousarray(features, dtype=np.float32)
-    D, H, W, _ = f.shape
+    D, H, W, cin = f.shape
+    if cin != NET_CIN:
+        raise ValueError(f"features must have {NET_CIN} channels")
     out = np.zeros((D, H, W), dtype=np.float32)
     fp = C.POINTER(C.c_float)
     _check(lib().doma_net_forward(params.ctypes.data_as(fp), params.size, D, H, W,
@@ -621,13 +633,15 @@ class Batch:
                "doma_batch_sample_count")
         return its[: n.value].tolist()
 
-    def get_sample(self, p: int, g: int, k: int):
+    def get_sample(self, p: int, g: int, k: int, with_prev: bool = False):
+        """(d_k, target) or, with_prev, (d_k, d_(k-1), target) of snapshot k."""
         N = self.problems[p].N
-        r, t = np.zeros(N, dtype=np.float32), np.zeros(N, dtype=np.float32)
+        r, rp, t = (np.zeros(N, dtype=np.float32) for _ in range(3))
         fp = C.POINTER(C.c_float)
         _check(lib().doma_batch_get_sample(self.h, p, g, k, r.ctypes.data_as(fp),
-                                           t.ctypes.data_as(fp)), "doma_batch_get_sample")
-        return r, t
+                                           rp.ctypes.data_as(fp), t.ctypes.data_as(fp)),
+               "doma_batch_get_sample")
+        return (r, rp, t) if with_prev else (r, t)
 
 
 class NetTrainer:
@@ -663,8 +677,9 @@ class NetTrainer:
         f = np.ascontiguousarray(features, dtype=np.float32)
         t = np.ascontiguousarray(targets, dtype=np.float32)
         n = self.batch * self.crop ** 3
-        if f.size != 3 * n or t.size != n:
-            raise ValueError("features must be [batch, crop, crop, crop, 3], targets [batch, crop^3]")
+        if f.size != NET_CIN * n or t.size != n:
+            raise ValueError(f"features must be [batch, crop, crop, crop, {NET_CIN}], "
+                             "targets [batch, crop^3]")
         return f, t
 
     def step(self, features, targets, lr: float) -> float:

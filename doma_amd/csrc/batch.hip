@@ -7,8 +7,8 @@
 // problem solved alone with the same kernel shape.
 //
 // Training data: every `sample_every` source iterations of the first multigroup
-// pass, the latest increment (the SI residual field) and the accumulated flux of
-// the group are kept on the device; when the group's iteration has converged
+// pass, the latest two increments (the SI residual history) and the accumulated
+// flux of the group are kept on the device; when the group's iteration has converged
 // each stored flux becomes the target phi_converged - phi_k (FP32); snapshots
 // of iterations that did not converge are dropped.  At most
 // `sample_cap` snapshots are kept per (problem, group): when the store is full,
@@ -33,9 +33,11 @@ struct SampleSet
   int stride = 1;           // snapshots at iterations stride * every * (j + 1)
   int count = 0;
   std::vector<int> its;     // SI iteration of each snapshot
-  float * d_res = nullptr;  // [cap][N]
-  double * d_phi = nullptr; // [cap][N]
-  float * d_tgt = nullptr;  // [cap][N]
+  float * d_res = nullptr;   // [cap][N] latest increment d_k
+  float * d_prev = nullptr;  // [cap][N] the increment before it, d_(k-1)
+  float * d_stage = nullptr; // [N] d_(k-1), staged one iteration before a snapshot
+  double * d_phi = nullptr;  // [cap][N]
+  float * d_tgt = nullptr;   // [cap][N]
   bool finalized = false;
 };
 
@@ -125,6 +127,8 @@ free_samples(doma_batch * b)
     for (auto & s : per)
     {
       dfree(s.d_res);
+      dfree(s.d_prev);
+      dfree(s.d_stage);
       dfree(s.d_phi);
       dfree(s.d_tgt);
     }
@@ -349,6 +353,8 @@ capture(doma_batch * b, int p, int g, int it, int every)
     {
       DOMA_HIP(hipMemcpyAsync(s.d_res + keep * N, s.d_res + j * N, N * sizeof(float),
                               hipMemcpyDeviceToDevice, b->stream));
+      DOMA_HIP(hipMemcpyAsync(s.d_prev + keep * N, s.d_prev + j * N, N * sizeof(float),
+                              hipMemcpyDeviceToDevice, b->stream));
       DOMA_HIP(hipMemcpyAsync(s.d_phi + keep * N, s.d_phi + j * N, N * sizeof(double),
                               hipMemcpyDeviceToDevice, b->stream));
       s.its[keep] = s.its[j];
@@ -362,16 +368,40 @@ capture(doma_batch * b, int p, int g, int it, int every)
   if (!s.d_res)
   {
     int rc;
-    if ((rc = dalloc(&s.d_res, b->sample_cap * N)) || (rc = dalloc(&s.d_phi, b->sample_cap * N)))
+    if ((rc = dalloc(&s.d_res, b->sample_cap * N)) || (rc = dalloc(&s.d_phi, b->sample_cap * N)) ||
+        (rc = dalloc(&s.d_prev, b->sample_cap * N)))
       return rc;
   }
   IterArgs a = iter_args(c, g);
   int rc = launch_capture(a, s.d_res + s.count * N, s.d_phi + s.count * N, b->stream);
   if (rc)
     return rc;
+  // d_(k-1) was staged after the previous iteration (every iteration before a
+  // multiple of `every` stages its increment).
+  DOMA_HIP(hipMemcpyAsync(s.d_prev + s.count * N, s.d_stage, N * sizeof(float),
+                          hipMemcpyDeviceToDevice, b->stream));
   s.its.push_back(it);
   ++s.count;
   return DOMA_OK;
+}
+
+// After iteration `it` of (p, g): keep its increment when the next iteration is a
+// multiple of `every` (a snapshot may then need it as d_(k-1)).
+int
+stage(doma_batch * b, int p, int g, int it, int every)
+{
+  if ((it + 1) % every != 0)
+    return DOMA_OK;
+  SampleSet & s = b->samples[p][g];
+  doma_ctx * c = b->ctx[p];
+  if (!s.d_stage)
+  {
+    int rc = dalloc(&s.d_stage, (size_t)c->N);
+    if (rc)
+      return rc;
+  }
+  double * unused = nullptr;
+  return launch_capture(iter_args(c, g), s.d_stage, unused, b->stream);
 }
 
 // Targets of (p, g) once its source iteration has ended.  Snapshots of an
@@ -627,9 +657,11 @@ doma_batch_solve(doma_batch * b, double fs_tol, int32_t fs_max_it, double mg_tol
     return rc;
   if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
     return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
-  if (sample_every < 0 || sample_cap < 0 || (sample_every > 0 && sample_cap < 2) ||
+  if (sample_every < 0 || sample_every == 1 || sample_cap < 0 ||
+      (sample_every > 0 && sample_cap < 2) ||
       sample_cap > 4096)
-    return fail(DOMA_ERR_INVALID, "sampling: every >= 0 and 2 <= cap <= 4096 when sampling");
+    return fail(DOMA_ERR_INVALID,
+                "sampling: every = 0 (off) or >= 2, and 2 <= cap <= 4096 when sampling");
   if (!b->configured && (rc = configure(b)))
     return rc;
   const double t0 = now_ms();
@@ -696,7 +728,9 @@ doma_batch_solve(doma_batch * b, double fs_tol, int32_t fs_max_it, double mg_tol
       const double r = b->h_res[4 * slot];
       ++s.si_it;
       ++s.st.sweeps;
-      if (sample_every > 0 && s.mg_it == 0 && (rc = capture(b, p, s.g, s.si_it, sample_every)))
+      if (sample_every > 0 && s.mg_it == 0 &&
+          ((rc = capture(b, p, s.g, s.si_it, sample_every)) ||
+           (rc = stage(b, p, s.g, s.si_it, sample_every))))
         return rc;
       if (s.si_it < fs_max_it && fs_tol < r)
       {
@@ -794,7 +828,7 @@ doma_batch_sample_count(const doma_batch * b, int32_t p, int32_t g, int32_t * co
 
 extern "C" int
 doma_batch_get_sample(doma_batch * b, int32_t p, int32_t g, int32_t k, float * residual,
-                      float * target)
+                      float * prev_residual, float * target)
 {
   if (!b || p < 0 || p >= b->P || b->samples.empty() || g < 0 || g >= b->ctx[p]->G)
     return fail(DOMA_ERR_INVALID, "doma_batch_get_sample: bad argument");
@@ -804,6 +838,9 @@ doma_batch_get_sample(doma_batch * b, int32_t p, int32_t g, int32_t k, float * r
   const size_t N = (size_t)b->ctx[p]->N;
   if (residual)
     DOMA_HIP(hipMemcpy(residual, s.d_res + k * N, N * sizeof(float), hipMemcpyDeviceToHost));
+  if (prev_residual)
+    DOMA_HIP(hipMemcpy(prev_residual, s.d_prev + k * N, N * sizeof(float),
+                       hipMemcpyDeviceToHost));
   if (target)
     DOMA_HIP(hipMemcpy(target, s.d_tgt + k * N, N * sizeof(float), hipMemcpyDeviceToHost));
   return DOMA_OK;

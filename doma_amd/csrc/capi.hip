@@ -700,8 +700,21 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
   // the corrected state; if that sweep's residual is not below the one before
   // the correction, the correction is undone (the sweep still counts).
   const bool learned = c->net.ready && c->net.every > 0 && c->dims == 3;
-  if (!learned && c->si_chunked)
+  const bool err_rule = c->stop_rule == DOMA_STOP_ERROR_ESTIMATE;
+  if (!learned && !err_rule && c->si_chunked)
     return si_loop_device(c, g, tol, max_it, its, res, conv, sweep_ms, sweeps, first_corr);
+  // Error-estimate rule: rho from ratios of consecutive plain iterations.
+  if ((int)c->rho_est.size() < c->G)
+    c->rho_est.assign(c->G, -1.0);
+  double rho_prev = c->rho_est[g], r_plain = -1.0;
+  auto not_done = [&](double r_) {
+    if (!err_rule)
+      return tol < r_;
+    if (r_ == 0.0)
+      return false;
+    const double rho = std::min(std::max(c->rho_est[g], rho_prev), 0.9999);
+    return rho < 0.0 || tol < r_ * rho / (1.0 - rho);
+  };
   IterArgs a = iter_args(c, g);
   if (learned)
     a.dcur = c->d_dcur;
@@ -743,6 +756,18 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       ++*sweeps;
     const double rn = c->h_res[0];
     ++it;
+    if (err_rule)
+    {
+      // A ratio counts only between two plain iterations (this sweep neither
+      // follows a correction nor a warm start's first increment).
+      const bool plain = !pending && !(it == 1 && first_corr);
+      if (plain && r_plain > 0.0)
+      {
+        rho_prev = c->rho_est[g];
+        c->rho_est[g] = rn / r_plain;
+      }
+      r_plain = plain ? rn : -1.0;
+    }
     if (!learned)
     {
       r = rn;
@@ -773,13 +798,15 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     std::swap(c->d_dlast, c->d_dcur);
     a.dcur = c->d_dcur;
     r = rn;
-    if (tol < r && it < max_it && it >= next_at)
+    if (not_done(r) && it < max_it && it >= next_at)
     {
       const double t0 = std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now().time_since_epoch())
                             .count();
-      if ((rc = net::net_apply(c->net, c->d_dlast, c->d_mat, c->d_sigt, c->d_sigs, c->G, g,
-                               c->d_corr, c->stream)))
+      // d_dlast holds the latest increment and d_dcur the one before: both come
+      // from plain iterations, since corrections are at least two sweeps apart.
+      if ((rc = net::net_apply(c->net, c->d_dlast, c->d_dcur, c->d_mat, c->d_sigt, c->d_sigs,
+                               c->G, g, c->d_corr, c->stream)))
         return rc;
       IterArgs ac = a;
       ac.corr = c->d_corr;
@@ -795,7 +822,7 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       pending = true;
       r_before = r;
     }
-  } while (it < max_it && tol < r);
+  } while (it < max_it && not_done(r));
   if (its)
     *its = it;
   if (res)
@@ -1338,6 +1365,7 @@ doma_solve_fixed_source(doma_ctx * c, double fs_tol, int32_t fs_max_it, double m
   if (!(fs_tol >= 0.0) || !(mg_tol >= 0.0) || fs_max_it < 1 || mg_max_it < 1)
     return fail(DOMA_ERR_INVALID, "tolerances must be >= 0 and iteration limits >= 1");
   c->transient = false;  // a steady-state solve ends any transient in progress
+  c->rho_est.assign(c->G, -1.0);  // contraction estimates belong to one solve
   const double t0 = now_ms();
   const size_t N = (size_t)c->N;
   // initializeSolve (TransportSolver3D.cpp:499-529): phi = 0.
@@ -1915,8 +1943,10 @@ doma_set_learned_correction(doma_ctx * c, const float * params, int64_t count, i
   int rc = check_ctx(c, false);
   if (rc)
     return rc;
-  if (every < 0)
-    return fail(DOMA_ERR_INVALID, "doma_set_learned_correction: every must be >= 0");
+  if (every < 0 || every == 1)
+    return fail(DOMA_ERR_INVALID,
+                "doma_set_learned_correction: every must be 0 (off) or >= 2 (the network reads "
+                "the last two increments of plain iterations)");
   if (every == 0)
   {
     net::net_free(c->net);
@@ -1958,10 +1988,11 @@ doma_net_forward(const float * params, int64_t count, int32_t D, int32_t H, int3
   const size_t V = (size_t)D * H * W;
   float * df = nullptr;
   float * dout = nullptr;
-  if (hipMalloc(&df, V * 3 * sizeof(float)) != hipSuccess ||
+  if (hipMalloc(&df, V * net::CIN * sizeof(float)) != hipSuccess ||
       hipMalloc(&dout, V * sizeof(float)) != hipSuccess)
     rc = fail(DOMA_ERR_DEVICE, "hipMalloc failed");
-  if (!rc && hipMemcpy(df, features, V * 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+  if (!rc && hipMemcpy(df, features, V * net::CIN * sizeof(float), hipMemcpyHostToDevice) !=
+                 hipSuccess)
     rc = fail(DOMA_ERR_DEVICE, "upload failed");
   if (!rc)
     rc = net::net_forward_features(n, df, dout, nullptr);
@@ -1973,6 +2004,19 @@ doma_net_forward(const float * params, int64_t count, int32_t D, int32_t H, int3
     (void)hipFree(dout);
   net::net_free(n);
   return rc;
+}
+
+extern "C" int
+doma_set_stopping_rule(doma_ctx * c, int32_t rule)
+{
+  int rc = check_ctx(c, false);
+  if (rc)
+    return rc;
+  if (rule != DOMA_STOP_RESIDUAL && rule != DOMA_STOP_ERROR_ESTIMATE)
+    return fail(DOMA_ERR_INVALID, "doma_set_stopping_rule: unknown rule");
+  c->stop_rule = rule;
+  c->rho_est.assign(c->G, -1.0);
+  return DOMA_OK;
 }
 
 extern "C" int

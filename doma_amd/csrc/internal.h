@@ -385,6 +385,10 @@ struct doma_ctx
   double * d_dcur = nullptr;   // [N] increment of the latest sweep
   double * d_keep = nullptr;   // [N] flux before the pending correction
   int warm_start = 0;          // start group iterations from the previous MG iterate
+  // Stopping rule of source iteration (DOMA_STOP_*) and, for the error-estimate
+  // rule, the latest contraction estimate per group (kept across multigroup passes).
+  int stop_rule = DOMA_STOP_RESIDUAL;
+  std::vector<double> rho_est;
 };
 
 namespace doma

@@ -397,7 +397,8 @@ capture_kernel(IterArgs a, float * res_out, double * phi_out)
     return;
   const double d = sum_parts(a.part, a.ndirsets, (size_t)a.N, c);
   res_out[c] = static_cast<float>(d);
-  phi_out[c] = a.phi[(size_t)a.g * a.N + c];
+  if (phi_out)
+    phi_out[c] = a.phi[(size_t)a.g * a.N + c];
 }
 
 // Targets of the stored snapshots once the group's iteration has converged:

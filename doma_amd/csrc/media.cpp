@@ -56,6 +56,15 @@ scatter_split(int rule, int G, int g, double * frac /* [G] */)
       frac[g + 1] = 0.3;
     return;
   }
+  if (rule == DOMA_SCATTER_INGROUP)
+  {
+    // Scattering stays in the group except a 2 % share to the next group; the
+    // last group keeps all of it.
+    frac[g] = g + 1 < G ? 0.98 : 1.0;
+    if (g + 1 < G)
+      frac[g + 1] = 0.02;
+    return;
+  }
   // DOMA_SCATTER_FULL
   if (g == G - 1)
   {

@@ -1,8 +1,9 @@
 // unet.h -- the learned source-iteration correction: a 3-level 3D U-Net on
-// BF16 MFMA (FP32 accumulate) that maps the latest SI increment of a group (the
-// SI residual field) plus the group's sigma_t and in-group scattering ratio to an
-// estimate of the remaining error phi_converged - phi.  Host-side description
-// shared by the C ABI, the kernels (unet.hip) and the Python trainer.
+// BF16 MFMA (FP32 accumulate) that maps the last two SI increments of a group
+// (the SI residual history) plus the group's sigma_t and in-group scattering
+// ratio to an estimate of the remaining error phi_converged - phi, in units of
+// the geometric-series extrapolation of the latest increment.  Host-side
+// description shared by the C ABI, the kernels (unet.hip) and the trainer.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,7 +17,9 @@ namespace net
 {
 // Channel widths of the three levels and the layer list.  Layers 0-9 are 3x3x3
 // convolutions (zero padding 1) followed by ReLU; layer 10 is the 1x1x1 head.
-constexpr int C1 = 12, C2 = 24, C3 = 32, CIN = 3;
+// Input channels, per cell: d_k / s, d_(k-1) / s (s = rms of d_k), log sigma_t,
+// log 1/(1 - c_gg + 1e-4).
+constexpr int C1 = 12, C2 = 24, C3 = 32, CIN = 4;
 constexpr int kLayers = 11;
 
 struct LayerSpec
@@ -76,7 +79,7 @@ struct Net
   uint16_t * d2a = nullptr, * d2b = nullptr;  // [V/8][24]
   uint16_t * d1a = nullptr, * d1b = nullptr;  // [V][16]
   double * red = nullptr;                     // reduction partials
-  double * scale = nullptr;                   // [1] rms of the input increment
+  double * scale = nullptr;                   // [2] rms of d_k, extrapolation gain g
 };
 
 // Packs the flat FP32 parameters into per-layer BF16 device layouts and sizes
@@ -84,12 +87,17 @@ struct Net
 int net_setup(Net & n, const float * params, int D, int H, int W);
 void net_free(Net & n);
 
-// corr[c] = s * U-Net(dlast / s, log sigma_t, log 1/(1 - c_gg + 1e-4)) /
-// (1 - c_gg + 1e-4) for group g, s = rms(dlast).  mat/sigt/sigs as in IterArgs.
-int net_apply(Net & n, const double * dlast, const int32_t * mat, const double * sigt,
-              const double * sigs, int G, int g, double * corr, hipStream_t s);
+// corr[c] = s * gain * U-Net(dlast / s, dprev / s, log sigma_t,
+// log 1/(1 - c_gg + 1e-4)) for group g, with s = rms(dlast), rho = rms(dlast) /
+// rms(dprev) clamped to [0, 0.999] and gain = rho / (1 - rho): the network
+// refines the geometric-series extrapolation gain * dlast of the remaining error.
+// dlast and dprev are the latest two increments of plain source iterations;
+// mat/sigt/sigs as in IterArgs.
+int net_apply(Net & n, const double * dlast, const double * dprev, const int32_t * mat,
+              const double * sigt, const double * sigs, int G, int g, double * corr,
+              hipStream_t s);
 
-// Forward pass on given features (testing): feat FP32 [V][3] -> raw network
+// Forward pass on given features (testing): feat FP32 [V][CIN] -> raw network
 // output FP32 [V] (no scale, no gain).
 int net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipStream_t s);
 
@@ -100,7 +108,7 @@ int net_forward_features(Net & n, const float * feat_dev, float * out_dev, hipSt
 struct Trainer;
 int trainer_create(int crop, int batch, const float * params, Trainer ** out);
 void trainer_free(Trainer * t);
-// feats [batch][crop^3][3], targets [batch][crop^3] on the host.  update =
+// feats [batch][crop^3][CIN], targets [batch][crop^3] on the host.  update =
 // false computes the loss and the gradient only (grad_out may be NULL).
 int trainer_run(Trainer * t, const float * feats, const float * targets, float lr, bool update,
                 float * loss, float * grad_out);

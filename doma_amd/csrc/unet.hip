@@ -15,8 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <new>
 #include <cstring>
+#include <new>
 
 #include "internal.h"
 #include "unet.h"
@@ -196,33 +196,46 @@ sumsq_kernel(const double * d, int N, double * red)
     red[blockIdx.x] = sh[0];
 }
 
+// scale[0] = rms of the first sum (sum d_k^2 over N cells), scale[1] = the
+// extrapolation gain rho / (1 - rho), rho = rms(d_k) / rms(d_(k-1)) clamped to
+// [0, 0.999] (red holds the two sets of block partial sums).
 __global__ __launch_bounds__(RT) void
 rms_final_kernel(const double * red, int nblocks, int N, double * scale)
 {
-  __shared__ double sh[RT];
-  double s = 0.0;
+  __shared__ double sh[2][RT];
+  double s0 = 0.0, s1 = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += RT)
-    s += red[b];
-  sh[threadIdx.x] = s;
+  {
+    s0 += red[b];
+    s1 += red[nblocks + b];
+  }
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = s1;
   __syncthreads();
   for (int k = RT / 2; k > 0; k >>= 1)
   {
     if ((int)threadIdx.x < k)
-      sh[threadIdx.x] += sh[threadIdx.x + k];
+    {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0)
   {
-    const double r = sqrt(sh[0] / N);
+    const double r = sqrt(sh[0][0] / N), rp = sqrt(sh[1][0] / N);
     scale[0] = r > 0.0 ? r : 1.0;
+    double rho = rp > 0.0 ? r / rp : 0.0;
+    rho = rho < 0.0 ? 0.0 : (rho > 0.999 ? 0.999 : rho);
+    scale[1] = rho / (1.0 - rho);
   }
 }
 
-// Network input of group g: the increment scaled by its rms, log sigma_t and
-// log 1/(1 - c_gg + 1e-4) with c_gg = sigma_s,gg / sigma_t.
+// Network input of group g: the last two increments scaled by the rms of the
+// latest, log sigma_t and log 1/(1 - c_gg + 1e-4) with c_gg = sigma_s,gg / sigma_t.
 __global__ void
-features_kernel(const double * d, const double * scale, const int32_t * mat, const double * sigt,
-                const double * sigs, int G, int g, int N, uint16_t * feat)
+features_kernel(const double * d, const double * dp, const double * scale, const int32_t * mat,
+                const double * sigt, const double * sigs, int G, int g, int N, uint16_t * feat)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N)
@@ -231,14 +244,12 @@ features_kernel(const double * d, const double * scale, const int32_t * mat, con
   const double st = sigt[(size_t)m * G + g];
   const double sgg = sigs[((size_t)m * G + g) * G + g];
   const double cgg = st > 0.0 ? sgg / st : 0.0;
-  const float f0 = (float)(d[c] / scale[0]);
-  const float f1 = (float)log(st > 1e-6 ? st : 1e-6);
-  const float f2 = (float)log(1.0 / (1.0 - cgg + 1e-4));
   uint16_t * o = feat + (size_t)c * 8;
-  o[0] = f2bf(f0);
-  o[1] = f2bf(f1);
-  o[2] = f2bf(f2);
-  for (int k = 3; k < 8; ++k)
+  o[0] = f2bf((float)(d[c] / scale[0]));
+  o[1] = f2bf((float)(dp[c] / scale[0]));
+  o[2] = f2bf((float)log(st > 1e-6 ? st : 1e-6));
+  o[3] = f2bf((float)log(1.0 / (1.0 - cgg + 1e-4)));
+  for (int k = CIN; k < 8; ++k)
     o[k] = 0;
 }
 
@@ -249,7 +260,7 @@ features_f32_kernel(const float * in, int N, uint16_t * feat)
   if (c >= N)
     return;
   for (int k = 0; k < 8; ++k)
-    feat[(size_t)c * 8 + k] = k < 3 ? f2bf(in[(size_t)c * 3 + k]) : 0;
+    feat[(size_t)c * 8 + k] = k < CIN ? f2bf(in[(size_t)c * CIN + k]) : 0;
 }
 
 // 1x1x1 head: y = w . h + b.  Raw output for the stand-alone forward pass.
@@ -270,21 +281,16 @@ head_raw_kernel(const uint16_t * h, int stride, int C, const float * wb, int N, 
     out[c] = head_value(h, stride, C, wb, c);
 }
 
-// Correction: e = scale * y / (1 - c_gg + 1e-4).  The network predicts the error
-// in units of the infinite-medium gain, which keeps its output O(1) from weakly
-// to strongly scattering cells.
+// Correction: e = s * gain * y, the network's refinement of the geometric-series
+// extrapolation gain * d_k of the remaining error.
 __global__ void
 head_corr_kernel(const uint16_t * h, int stride, int C, const float * wb, const double * scale,
-                 const int32_t * mat, const double * sigt, const double * sigs, int G, int g,
                  int N, double * out)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N)
     return;
-  const int m = mat[c];
-  const double st = sigt[(size_t)m * G + g];
-  const double cgg = st > 0.0 ? sigs[((size_t)m * G + g) * G + g] / st : 0.0;
-  out[c] = scale[0] * (double)head_value(h, stride, C, wb, c) / (1.0 - cgg + 1e-4);
+  out[c] = scale[0] * scale[1] * (double)head_value(h, stride, C, wb, c);
 }
 
 inline unsigned
@@ -478,29 +484,30 @@ net_setup(Net & n, const float * params, int D, int H, int W)
       (rc = dalloc0(&n.bb, V4 * pad8(C3))) || (rc = dalloc0(&n.d2a, V2 * pad8(C2))) ||
       (rc = dalloc0(&n.d2b, V2 * pad8(C2))) || (rc = dalloc0(&n.d1a, V * pad8(C1))) ||
       (rc = dalloc0(&n.d1b, V * pad8(C1))) || (rc = dalloc0(&n.red, 1024)) ||
-      (rc = dalloc0(&n.scale, 1)))
+      (rc = dalloc0(&n.scale, 2)))
     return rc;
   n.ready = true;
   return DOMA_OK;
 }
 
 int
-net_apply(Net & n, const double * dlast, const int32_t * mat, const double * sigt,
-          const double * sigs, int G, int g, double * corr, hipStream_t s)
+net_apply(Net & n, const double * dlast, const double * dprev, const int32_t * mat,
+          const double * sigt, const double * sigs, int G, int g, double * corr, hipStream_t s)
 {
   if (!n.ready)
     return fail(DOMA_ERR_STATE, "the learned correction has no weights");
   const int N = n.D * n.H * n.W;
-  const int nb = 1024;
+  const int nb = 512;  // two sets of partial sums in n.red (1024 doubles)
   hipLaunchKernelGGL(sumsq_kernel, dim3(nb), dim3(RT), 0, s, dlast, N, n.red);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(nb), dim3(RT), 0, s, dprev, N, n.red + nb);
   hipLaunchKernelGGL(rms_final_kernel, dim3(1), dim3(RT), 0, s, n.red, nb, N, n.scale);
-  hipLaunchKernelGGL(features_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, dlast, n.scale, mat,
-                     sigt, sigs, G, g, N, n.feat);
+  hipLaunchKernelGGL(features_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, dlast, dprev, n.scale,
+                     mat, sigt, sigs, G, g, N, n.feat);
   int rc = body(n, s);
   if (rc)
     return rc;
   hipLaunchKernelGGL(head_corr_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
-                     n.layer[10].bias, n.scale, mat, sigt, sigs, G, g, N, corr);
+                     n.layer[10].bias, n.scale, N, corr);
   return DOMA_OK;
 }
 
@@ -891,7 +898,7 @@ struct Trainer
   float * v = nullptr;
   PackedLayer bwd[kLayers];  // data-gradient weights (layers 1-9)
   float * zero = nullptr;     // [64]
-  float * fin = nullptr;      // [V][3]
+  float * fin = nullptr;      // [V][CIN]
   float * y = nullptr;        // [V]
   float * pred = nullptr;     // [V]
   float * dpred = nullptr;    // [V]
@@ -1182,7 +1189,7 @@ trainer_create(int crop, int batch, const float * params, Trainer ** out)
   if (!rc)
   {
     if ((rc = alloc0(&t->params, P)) || (rc = alloc0(&t->grads, P)) || (rc = alloc0(&t->m, P)) ||
-        (rc = alloc0(&t->v, P)) || (rc = alloc0(&t->zero, 64)) || (rc = alloc0(&t->fin, V * 3)) ||
+        (rc = alloc0(&t->v, P)) || (rc = alloc0(&t->zero, 64)) || (rc = alloc0(&t->fin, V * CIN)) ||
         (rc = alloc0(&t->y, V)) || (rc = alloc0(&t->pred, V)) || (rc = alloc0(&t->dpred, V)) ||
         (rc = alloc0(&t->dyA, V * 64)) || (rc = alloc0(&t->dyB, V * 64)) ||
         (rc = alloc0(&t->gtmp, V * 64)) || (rc = alloc0(&t->acc1, V * 16)) ||
@@ -1229,7 +1236,7 @@ trainer_run(Trainer * t, const float * feats, const float * targets, float lr, b
 {
   const size_t V = (size_t)t->nb * t->C * t->C * t->C;
   hipStream_t s = nullptr;
-  DOMA_HIP(hipMemcpy(t->fin, feats, V * 3 * sizeof(float), hipMemcpyHostToDevice));
+  DOMA_HIP(hipMemcpy(t->fin, feats, V * CIN * sizeof(float), hipMemcpyHostToDevice));
   DOMA_HIP(hipMemcpy(t->y, targets, V * sizeof(float), hipMemcpyHostToDevice));
   int rc;
   if ((rc = forward(*t, s)) || (rc = backward(*t, s)))

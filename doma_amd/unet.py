@@ -3,19 +3,18 @@ layout, input features, and the trainer.
 
 The product path evaluates the network with the hand-written BF16 MFMA kernels
 of the native library (``doma_set_learned_correction``); this module is the
-tooling around it.  ``UNet`` is the same architecture in PyTorch, used to train
-the weights (autograd, FP32) on the (residual, converged - current flux) pairs
-the batched solver records, and as the FP32 reference that the native kernels
-are tested against.  Parameters cross the boundary as one flat FP32 vector,
-per layer W[cout][tap][cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then
-bias[cout], in the layer order below.
+tooling around it.  ``UNet`` is the same architecture in PyTorch, the FP32
+reference that the native kernels are tested against.  Parameters cross the
+boundary as one flat FP32 vector, per layer W[cout][tap][cin] (tap = (dz+1)*9 +
+(dy+1)*3 + (dx+1)) then bias[cout], in the layer order below.
 
-Input channels, per cell, for group g: the latest SI increment divided by its
-rms s over the grid, log sigma_t, and log(1 / (1 - c + 1e-4)) with c the
-in-group scattering ratio sigma_s,gg / sigma_t.  Output y: the remaining error
-phi_converged - phi = s * y / (1 - c + 1e-4), i.e. in units of the increment
-times the infinite-medium gain, which keeps y O(1) from weakly to strongly
-scattering cells.
+Input channels, per cell, for group g: the last two increments of plain source
+iterations d_k and d_(k-1), both divided by s = rms(d_k) over the grid, log
+sigma_t, and log(1 / (1 - c + 1e-4)) with c the in-group scattering ratio
+sigma_s,gg / sigma_t.  Output y: the remaining error phi_converged - phi_k =
+s * g * y with g = rho / (1 - rho), rho = rms(d_k) / rms(d_(k-1)) clamped to
+0.999 -- the network refines the geometric-series extrapolation g * d_k, which
+is exact when one mode dominates the iteration, into a per-cell estimate.
 """
 from __future__ import annotations
 
@@ -23,7 +22,8 @@ import math
 
 import numpy as np
 
-C1, C2, C3, CIN = 12, 24, 32, 3
+C1, C2, C3, CIN = 12, 24, 32, 4
+RHO_MAX = 0.999
 # (cin_a, cin_b, cout, taps): cin_b channels come from the coarser level, upsampled.
 SPEC = [(CIN, 0, C1, 27), (C1, 0, C1, 27), (C1, 0, C2, 27), (C2, 0, C2, 27), (C2, 0, C3, 27),
         (C3, 0, C3, 27), (C2, C3, C2, 27), (C2, 0, C2, 27), (C1, C2, C1, 27), (C1, 0, C1, 27),
@@ -34,13 +34,24 @@ def param_count() -> int:
     return sum(co * t * (a + b) + co for a, b, co, t in SPEC)
 
 
-def features(dlast, sigt, cgg):
-    """Network input [D, H, W, 3] from the increment and the group's cross sections."""
+def gain(dlast, dprev):
+    """(s, g): s = rms(d_k), g = rho / (1 - rho) with rho = rms(d_k) / rms(d_(k-1))."""
     d = np.asarray(dlast, dtype=np.float64)
+    dp = np.asarray(dprev, dtype=np.float64)
     s = math.sqrt(float((d * d).mean())) or 1.0
-    f = np.stack([d / s, np.log(np.maximum(sigt, 1e-6)), np.log(1.0 / (1.0 - cgg + 1e-4))],
-                 axis=-1)
-    return f.astype(np.float32), s
+    sp = math.sqrt(float((dp * dp).mean()))
+    rho = min(max(s / sp if sp > 0 else 0.0, 0.0), RHO_MAX)
+    return s, rho / (1.0 - rho)
+
+
+def features(dlast, dprev, sigt, cgg):
+    """Network input [D, H, W, 4] from the last two increments and the group's
+    cross sections; returns (features, s, g)."""
+    s, g = gain(dlast, dprev)
+    f = np.stack([np.asarray(dlast, dtype=np.float64) / s,
+                  np.asarray(dprev, dtype=np.float64) / s,
+                  np.log(np.maximum(sigt, 1e-6)), np.log(1.0 / (1.0 - cgg + 1e-4))], axis=-1)
+    return f.astype(np.float32), s, g
 
 
 def _torch():
@@ -112,7 +123,8 @@ def bf16_round(flat):
 
 
 def collect_samples(batch, problems, max_per_group=None):
-    """(features [D,H,W,3], scaled target [D,H,W]) pairs from a solved Batch."""
+    """(features [D,H,W,4], scaled target [D,H,W]) pairs from a solved Batch: the
+    target is (phi_converged - phi_k) / (s g)."""
     out = []
     for i, p in enumerate(problems):
         n = p.nx
@@ -122,10 +134,10 @@ def collect_samples(batch, problems, max_per_group=None):
             st = p.sigt[p.mat, g]
             cgg = np.where(st > 0, p.sigs[p.mat, g, g] / np.where(st > 0, st, 1.0), 0.0)
             for k in ks:
-                r, t = batch.get_sample(i, g, k)
-                c3 = cgg.reshape(n, n, n)
-                f, s = features(r.reshape(n, n, n), st.reshape(n, n, n), c3)
-                y = t.reshape(n, n, n) * (1.0 - c3 + 1e-4) / s
+                r, rp, t = batch.get_sample(i, g, k, with_prev=True)
+                f, s, gn = features(r.reshape(n, n, n), rp.reshape(n, n, n), st.reshape(n, n, n),
+                                    cgg.reshape(n, n, n))
+                y = t.reshape(n, n, n) / (s * gn if gn > 0 else s)
                 out.append((f, y.astype(np.float32)))
     return out
 
@@ -182,6 +194,32 @@ def train(samples, steps=2000, crops=(16, 32), batch_size=8, lr=1e-3, seed=0, de
     return flatten(model.cpu()), float(np.mean(errs)) if errs else float("nan")
 
 
+def extrapolation_params() -> np.ndarray:
+    """Hand-set parameters whose output is y = d_k / s (input channel 0) up to
+    BF16 rounding, so that the correction is exactly the geometric-series
+    extrapolation g * d_k: the first layer splits the channel into relu(x) and
+    relu(-x), the full-resolution layers and the decoder's skip path carry both
+    (centre tap), and the head takes their difference; the coarse levels get
+    zero weights.  The ablation that isolates what the trained network adds."""
+    flat = np.zeros(param_count(), dtype=np.float32)
+    off = 0
+    views = []
+    for a, b, co, t in SPEC:
+        ci = a + b
+        w = flat[off:off + co * t * ci].reshape(co, t, ci)
+        views.append(w)
+        off += co * t * ci + co
+    centre = 13  # tap (dz, dy, dx) = (0, 0, 0)
+    views[0][0, centre, 0] = 1.0   # enc1a: +x
+    views[0][1, centre, 0] = -1.0  #        -x
+    for layer in (1, 8, 9):        # enc1b, dec1a (skip channels 0, 1), dec1b
+        views[layer][0, centre, 0] = 1.0
+        views[layer][1, centre, 1] = 1.0
+    views[10][0, 0, 0] = 1.0       # head: relu(x) - relu(-x)
+    views[10][0, 0, 1] = -1.0
+    return flat
+
+
 def init_params(seed=0):
     """Fresh parameters without PyTorch: per layer W, b ~ U(-1/sqrt(fan_in),
     1/sqrt(fan_in)) (the default initialisation of a PyTorch Conv3d)."""
@@ -214,7 +252,7 @@ def train_native(samples, steps=3000, crop=32, batch_size=8, lr=1e-3, seed=0, lo
     c = crop
     with capi.NetTrainer(flat, crop=c, batch=batch_size) as tr:
         for step in range(steps):
-            xb = np.empty((batch_size, c, c, c, 3), dtype=np.float32)
+            xb = np.empty((batch_size, c, c, c, CIN), dtype=np.float32)
             yb = np.empty((batch_size, c, c, c), dtype=np.float32)
             for k in range(batch_size):
                 f, t = fit[int(rng.integers(len(fit)))]
@@ -237,34 +275,51 @@ def train_native(samples, steps=3000, crop=32, batch_size=8, lr=1e-3, seed=0, lo
 
 
 def training_problems(seed=20261210, small=False):
-    """The batched data-generation set: uneven cubes with 1/2/4 groups and
-    c ~ U[0.9, 0.99] (down-scatter), 8-group full-matrix high-scattering media
-    (c ~ U[0.95, 0.995], log-uniform sigma_t in [0.5, 10]), and one-group media
-    whose in-group scattering ratio reaches 0.995 (slow, diffusive SI)."""
+    """The batched data-generation set: uneven cubes (32, 48, 64 cells of 1 cm,
+    8^3-cell blocks) of high-scattering media in the regime the correction
+    serves -- sigma_t log-uniform in [0.5, 10], c ~ U[c_lo, 0.995] per (block,
+    group) with c_lo in {0.9, 0.95}, 98 % of it in-group, 1 or 2 groups,
+    sources in 5 or 10 % of the blocks -- diamond difference, 128 directions."""
     from . import capi, workloads
     from .capi import MediaSpec
 
-    k = 4 if small else 1
-    probs = workloads.batched_uneven(128 // k, seed=seed, sizes=(16, 24, 32, 48))
-    rng = np.random.default_rng(seed + 1)
-    for b in range(32 // k):
-        spec = MediaSpec(n=32, block=8, groups=8, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
-                         c_lo=0.95, c_hi=0.995, scatter_rule=capi.SCATTER_FULL,
+    rng = np.random.default_rng(seed)
+    probs = []
+    for b in range(96 // (4 if small else 1)):
+        n = int(rng.choice([32, 48, 64]))
+        groups = int(rng.choice([1, 2]))
+        c_lo = float(rng.choice([0.9, 0.95]))
+        frac = float(rng.choice([0.05, 0.1]))
+        spec = MediaSpec(n=n, block=8, groups=groups, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
+                         c_lo=c_lo, c_hi=0.995, scatter_rule=capi.SCATTER_INGROUP,
                          source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
-                         source_fraction=0.05, cell_width=1.0, seed=seed * 7 + b)
-        probs.append(workloads.block_media_problem(32, 8, 4, 4, capi.SCHEME_TW_DD, spec,
-                                                   f"full8_{b}"))
-    for b in range(32 // k):
-        n = int(rng.choice([24, 32, 48]))
-        lo = float(rng.choice([0.5, 0.8, 0.95]))
-        spec = MediaSpec(n=n, block=8, groups=1, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
-                         c_lo=lo, c_hi=0.995, scatter_rule=capi.SCATTER_DOWN1,
-                         source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
-                         source_fraction=0.1, cell_width=1.0, seed=seed * 11 + b)
-        # one group: the whole scattering ratio stays in-group
-        probs.append(workloads.block_media_problem(n, 1, 4, 4, capi.SCHEME_TW_DD, spec,
-                                                   f"diffusive_{b}"))
+                         source_fraction=frac, cell_width=1.0, seed=seed * 1000 + b)
+        probs.append(workloads.block_media_problem(n, groups, 4, 4, capi.SCHEME_DD, spec,
+                                                   f"train_{b}_{n}c_{groups}g"))
     return probs
+
+
+def generate_samples(probs, fs_tol=1e-10, sample_every=5, sample_cap=16, log=None):
+    """Run the problems as one batch (one persistent sweep launch per round) to a
+    tight tolerance, so the targets phi_converged - phi_k are accurate, and
+    collect the snapshots."""
+    from .capi import Batch
+
+    samples = []
+    groups = {}
+    for p in probs:  # one batch per quadrature (a batch shares one)
+        groups.setdefault(p.dirs, []).append(p)
+    for dirs, ps in groups.items():
+        with Batch(ps) as b:
+            b.autotune()
+            stats, info = b.solve(fs_tol, 20000, fs_tol, 4, sample_every=sample_every,
+                                  sample_cap=sample_cap)
+            ok = sum(1 for s in stats if s["success"])
+            if log:
+                log(f"data: {len(ps)} problems, {info['rounds']} rounds, {ok} converged, "
+                    f"{info['samples']} samples, {info['elapsed_ms'] / 1e3:.1f} s")
+            samples += collect_samples(b, ps)
+    return samples
 
 
 def save_samples(path, samples):
@@ -283,61 +338,53 @@ def load_samples(path):
 
 def main(argv=None):
     """Training data and weights for the correction:
+      python -m doma_amd.unet fit OUT.npy [--steps N] [--small]
       python -m doma_amd.unet data SAMPLES.npz [--small] [--sample-cap K]
-      python -m doma_amd.unet train SAMPLES.npz OUT.npy [--steps N] [--native]
-    --native trains with the library's own kernels.  Without it PyTorch trains
-    the twin network; then data and training must run in separate processes
-    (the library and PyTorch each load their own HIP runtime, and only one of
-    them can own the GPU in a process)."""
+      python -m doma_amd.unet train SAMPLES.npz OUT.npy [--steps N] [--torch]
+    fit generates the data with the batched solver and trains with the library's
+    own kernels in one process.  --torch trains the PyTorch twin instead; data and
+    PyTorch training must then run in separate processes (the library and
+    PyTorch each load their own HIP runtime, and only one of them can own the GPU
+    in a process)."""
     import argparse
     import sys
     import time
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("phase", choices=["data", "train"])
-    ap.add_argument("samples")
+    ap.add_argument("phase", choices=["fit", "data", "train"])
+    ap.add_argument("path")
     ap.add_argument("out", nargs="?")
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--small", action="store_true")
-    ap.add_argument("--sample-cap", type=int, default=8)
-    ap.add_argument("--native", action="store_true",
-                    help="train with the library's hand-written kernels instead of PyTorch")
+    ap.add_argument("--sample-cap", type=int, default=16)
+    ap.add_argument("--torch", action="store_true",
+                    help="train the PyTorch twin instead of the library's kernels")
     args = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
     t0 = time.time()
-    if args.phase == "data":
-        from .capi import Batch
-
-        probs = training_problems(small=args.small)
-        groups = {}
-        for p in probs:  # one batch per quadrature (a batch shares one)
-            groups.setdefault(p.dirs, []).append(p)
-        samples = []
-        for dirs, ps in groups.items():
-            log(f"data: batch of {len(ps)} problems, {dirs} directions per octant")
-            with Batch(ps) as b:
-                b.autotune()
-                stats, info = b.solve(1e-6, 5000, 1e-6, 60, sample_every=5,
-                                      sample_cap=args.sample_cap)
-                ok = sum(1 for s in stats if s["success"])
-                log(f"data: {info['rounds']} rounds, {ok}/{len(ps)} converged, "
-                    f"{info['samples']} samples, {info['elapsed_ms'] / 1e3:.1f} s")
-                samples += collect_samples(b, ps)
-        save_samples(args.samples, samples)
-        log(f"data: {len(samples)} pairs written to {args.samples} in {time.time() - t0:.0f} s")
-        return
-    samples = load_samples(args.samples)
-    log(f"train: {len(samples)} pairs" + (" (native kernels)" if args.native else " (PyTorch)"))
-    if args.native:
-        flat, held_err = train_native(samples, steps=args.steps, log=log)
+    if args.phase in ("fit", "data"):
+        samples = generate_samples(training_problems(small=args.small),
+                                   sample_cap=args.sample_cap, log=log)
+        log(f"data: {len(samples)} pairs in {time.time() - t0:.0f} s")
+        if args.phase == "data":
+            save_samples(args.path, samples)
+            log(f"data: written to {args.path}")
+            return
+        out = args.path
     else:
+        samples = load_samples(args.path)
+        out = args.out
+        log(f"train: {len(samples)} pairs")
+    if args.torch:
         flat, held_err = train(samples, steps=args.steps, log=log)
-    np.save(args.out, flat)
+    else:
+        flat, held_err = train_native(samples, steps=args.steps, log=log)
+    np.save(out, flat)
     log(f"held-out relative L2 error of the predicted correction: {held_err:.3f}")
-    log(f"saved {args.out} ({flat.size} parameters) after {time.time() - t0:.0f} s")
+    log(f"saved {out} ({flat.size} parameters) after {time.time() - t0:.0f} s")
 
 
 if __name__ == "__main__":

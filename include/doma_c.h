@@ -257,10 +257,14 @@ int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
 int64_t doma_net_param_count(void);
 
 /* Enable the correction on a 3D context (extents divisible by 4): every `every`
- * source iterations the network maps the latest increment, sigma_t and the
- * in-group scattering ratio of the group to a flux correction (BF16 on MFMA,
- * FP32 accumulate); the correction is undone if the next residual is not below
- * the one before it.  every = 0 disables it; params may then be NULL. */
+ * source iterations (every >= 2) the network maps the last two increments of
+ * the group's plain iterations d_k, d_(k-1) (each divided by rms(d_k)), sigma_t
+ * and the in-group scattering ratio to a flux correction s * g * y (BF16 on
+ * MFMA, FP32 accumulate), s = rms(d_k), g = rho / (1 - rho), rho = rms(d_k) /
+ * rms(d_(k-1)) clamped to 0.999: the network refines the geometric-series
+ * extrapolation g * d_k of the remaining error.  The correction is undone if the
+ * next residual is not below the one before it.  every = 0 disables it; params
+ * may then be NULL. */
 int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t count, int32_t every);
 
 /* Accelerated multigroup iteration (no reference counterpart; off by default):
@@ -269,7 +273,20 @@ int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t coun
  * TransportSolver3D.cpp:399-400).  2D/3D. */
 int doma_set_warm_start(doma_ctx *ctx, int32_t enable);
 
-/* Stand-alone forward pass (diagnostics and tests): features[V][3] (FP32, V =
+/* Stopping rule of source iteration (no reference counterpart for rule 1).
+ * DOMA_STOP_RESIDUAL (default): the reference's rule, stop once the relative
+ * increment r_k = ||dphi_k|| / ||phi_k|| <= tol (TransportSolver3D.cpp:458-497).
+ * DOMA_STOP_ERROR_ESTIMATE: stop once r_k * rho / (1 - rho) <= tol, where rho is
+ * the larger of the last two ratios r_k / r_(k-1) of consecutive plain source
+ * iterations (no correction in between) -- the geometric-series estimate of the
+ * remaining error phi_converged - phi_k relative to phi_k; the group keeps its
+ * estimate across multigroup passes.  Rule 1 runs the iteration with one host
+ * round trip per sweep. */
+#define DOMA_STOP_RESIDUAL 0
+#define DOMA_STOP_ERROR_ESTIMATE 1
+int doma_set_stopping_rule(doma_ctx *ctx, int32_t rule);
+
+/* Stand-alone forward pass (diagnostics and tests): features[V][4] (FP32, V =
  * D*H*W, x fastest) -> out[V]. */
 int doma_net_forward(const float *params, int64_t count, int32_t D, int32_t H, int32_t W,
                      const float *features, float *out);
@@ -278,7 +295,7 @@ int doma_net_forward(const float *params, int64_t count, int32_t D, int32_t H, i
  * crops of edge `crop` (multiple of 4): BF16 MFMA forward and data gradients,
  * FP32 weight gradients, loss sum (p - y)^2 / sum y^2, global-norm clipping at
  * 1, Adam (0.9, 0.999, 1e-8) on FP32 master weights.  features are
- * [batch][crop^3][3] and targets [batch][crop^3] (x fastest), on the host. */
+ * [batch][crop^3][4] and targets [batch][crop^3] (x fastest), on the host. */
 typedef struct doma_net_trainer doma_net_trainer;
 int doma_net_trainer_create(int32_t crop, int32_t batch, const float *params, int64_t count,
                             doma_net_trainer **out);
@@ -332,9 +349,10 @@ int doma_batch_get_flux(doma_batch *b, int32_t p, double *phi);
 /* Snapshots kept for (p, g) and their iteration numbers (iterations may be NULL). */
 int doma_batch_sample_count(const doma_batch *b, int32_t p, int32_t g, int32_t *count,
                             int32_t *iterations);
-/* Snapshot k of (p, g): residual[N] = latest SI increment, target[N] = phi_conv - phi_k. */
+/* Snapshot k of (p, g): residual[N] = the latest SI increment d_k, prev_residual[N]
+ * = the increment before it d_(k-1), target[N] = phi_conv - phi_k (any may be NULL). */
 int doma_batch_get_sample(doma_batch *b, int32_t p, int32_t g, int32_t k, float *residual,
-                          float *target);
+                          float *prev_residual, float *target);
 
 /* ------------------------------------------------------------------------- */
 /* Input decks (host C++ front end).                                           */
@@ -393,6 +411,9 @@ int doma_gl_quadrature(int32_t n, double *mu, double *w);
                                  the last group keeps only its in-group part */
 #define DOMA_SCATTER_FULL 1   /* in-group plus down-scatter to every lower group,
                                  up-scatter from the last group into the one above */
+#define DOMA_SCATTER_INGROUP 2 /* sigma_s[g][g] = 0.98 c sigma_t, [g+1][g] = 0.02 c sigma_t;
+                                 the last group keeps all of c in-group (down-scatter
+                                 only: the in-group iterations dominate) */
 #define DOMA_SOURCE_CENTRAL 0 /* unit source in group 0 on the cube [src_lo, src_hi)^3 */
 #define DOMA_SOURCE_BLOCKS 1  /* Q_g ~ U[0,1] in a seeded fraction of blocks */
 

@@ -1,8 +1,8 @@
 // ref_harness.cpp -- drives the reference's own compiled TransportSolver2D/3D
 // units (built from /root/reference/src by oracle/Makefile, target "ref") to
-// write golden vectors for the oracle and the HIP path.  TEST INFRASTRUCTURE:
-// it runs only in the build container, never on the GPU host and never inside
-// the product.
+// write golden vectors for the oracle and the HIP path, and to time the
+// reference's own OpenMP solve as the benchmark's CPU baseline.  TEST
+// INFRASTRUCTURE: never part of the product.
 //
 // The harness fills InputParameters programmatically (the reference's XML
 // reader needs pugixml, which is absent): every cell becomes its own mesh
@@ -10,7 +10,9 @@
 // BrickMesh2D/3D (BrickMesh3D.cpp:8-102) expands to exactly the requested cell
 // widths and material map.  The scalar flux is read back through an explicit
 // specialisation of TransportSolver3D/2D, which BrickMesh3D/2D declare as
-// friends for every template argument (BrickMesh3D.h:46-47).
+// friends for every template argument (BrickMesh3D.h:46-47).  The log ends with
+// the wall time of solveFixedSource ("solve_seconds <t>"), which the benchmark's
+// CPU baseline reads when it times the reference's own OpenMP solve.
 //
 // Problem file (little endian):
 //   int32 dims, nx, ny, nz, G, scheme, M, np, na, fs_max_it, mg_max_it, nthreads
@@ -24,6 +26,7 @@
 #include "TransportSolver2D.h"
 #include "TransportSolver3D.h"
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -132,6 +135,15 @@ main(int argc, char ** argv)
   auto * old = std::cout.rdbuf(log.rdbuf());
   bool ok = false;
   std::vector<double> phi(static_cast<size_t>(G) * N, 0.0);
+  // Wall time of solveFixedSource alone (mesh construction excluded), written to
+  // the log as "solve_seconds <t>" for the CPU baseline.
+  double solve_s = 0.0;
+  auto t_solve = [&solve_s](auto && solve) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool r = solve();
+    solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+  };
   if (dims == 3)
   {
     std::array<BoundaryCondition, 6u> bcs;
@@ -141,12 +153,12 @@ main(int argc, char ** argv)
     if (scheme == 1)
     {
       TransportSolver3D<TWDiamondDifference3D> s(mesh, params, true, T);
-      ok = s.solveFixedSource("");
+      ok = t_solve([&] { return s.solveFixedSource(""); });
     }
     else
     {
       TransportSolver3D<DiamondDifference3D> s(mesh, params, true, T);
-      ok = s.solveFixedSource("");
+      ok = t_solve([&] { return s.solveFixedSource(""); });
     }
     const auto & cells = TransportSolver3D<RefProbe>::cells(mesh);
     for (size_t c = 0; c < N; ++c)
@@ -162,12 +174,12 @@ main(int argc, char ** argv)
     if (scheme == 1)
     {
       TransportSolver2D<TWDiamondDifference2D> s(mesh, params, true, T);
-      ok = s.solveFixedSource("");
+      ok = t_solve([&] { return s.solveFixedSource(""); });
     }
     else
     {
       TransportSolver2D<DiamondDifference2D> s(mesh, params, true, T);
-      ok = s.solveFixedSource("");
+      ok = t_solve([&] { return s.solveFixedSource(""); });
     }
     const auto & cells = TransportSolver2D<RefProbe>::cells(mesh);
     for (size_t c = 0; c < N; ++c)
@@ -188,6 +200,6 @@ main(int argc, char ** argv)
   o.write(reinterpret_cast<const char *>(phi.data()),
           static_cast<std::streamsize>(phi.size() * sizeof(double)));
   std::ofstream lg(std::string(argv[2]) + ".log");
-  lg << log.str();
+  lg << log.str() << "solve_seconds " << solve_s << "\n";
   return 0;
 }

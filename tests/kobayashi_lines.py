@@ -15,10 +15,15 @@ REC = json.loads((Path(__file__).resolve().parent / "golden" / "kobayashi1_recor
                  .read_text())
 LINES = ("1a", "1b", "1c")
 # Reproduction bar: the recorded values carry 6 significant digits (half a unit
-# of the last digit is <= 5e-6 relative) and come from a run stopped at a 1e-7
+# of the last digit is <= 5e-6 relative) and come from runs stopped near a 1e-7
 # scattering residual; a diamond-difference solve to 1e-7 here lands within
-# 1e-5 of every recorded value at 20^3, 40^3 and 60^3 cells.
+# 1e-5 of every recorded value.  One recorded run (np = na = 2 on 80^3 cells)
+# stopped one sweep later than this one: its far-corner values on line 1b, five
+# orders of magnitude below the line's largest, match the 21st iterate and
+# differ from the 20th by 8e-5 relative; an absolute floor of 1e-9 of the line's
+# largest value covers that.
 KAT_RTOL = 1.5e-5
+KAT_FLOOR = 1e-9
 
 
 def recorded(line: str, angle: int, refinement: int) -> np.ndarray:
@@ -37,10 +42,12 @@ def line_values(p, phi, line: str) -> np.ndarray:
 
 
 def worst(p, phi, angle: int, refinement: int) -> dict:
-    """Largest relative deviation from the recorded row, per line."""
+    """Largest deviation from the recorded row per line, in units of the bar
+    KAT_RTOL |recorded| + KAT_FLOOR max |recorded| (<= 1 passes)."""
     out = {}
     for line in LINES:
         rec = recorded(line, angle, refinement)
         got = line_values(p, phi, line)
-        out[line] = float(np.max(np.abs(got - rec) / np.abs(rec)))
+        bar = KAT_RTOL * np.abs(rec) + KAT_FLOOR * np.abs(rec).max()
+        out[line] = float(np.max(np.abs(got - rec) / bar))
     return out

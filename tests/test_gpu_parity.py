@@ -1,11 +1,14 @@
 """HIP path (through the C ABI) against the reference's golden vectors and the
 CPU oracle.  Bar: FP64 scalar flux within 1e-12 relative (max |d phi| / max |phi|
-per group) with identical SI/MG iteration counts; FP32-sweep mode within 1e-4
-of the FP64 result, with the measured error printed."""
+per group) with identical SI/MG iteration counts, and per cell within 1e-12 or
+20x the reference's own per-cell spread across thread counts; FP32-sweep mode
+within 1e-5 of the reference's FP64 result, with the measured error printed."""
+import json
+
 import numpy as np
 import pytest
 
-from conftest import golden_names, load_golden, rel_err, solve_args
+from conftest import GOLDEN, golden_names, load_golden, rel_err, solve_args
 from doma_amd import capi, workloads
 from doma_amd.capi import Context
 from oracle import binding as ob
@@ -13,6 +16,7 @@ from oracle import binding as ob
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
+PERCELL_SPREAD = json.loads((GOLDEN / "percell_spread.json").read_text())
 
 
 def _solve(p, z, precision=capi.FP64, **tuning):
@@ -31,11 +35,28 @@ def test_fixed_source_matches_reference(name, exact):
     ref = z["phi"] if "phi" in z.files else None
     if ref is None:
         idx = z["sample_idx"]
-        err = rel_err(phi[:, idx], z["phi_sample"])
+        got, ref = phi[:, idx], z["phi_sample"]
     else:
-        err = rel_err(phi, ref)
+        got, ref = phi, ref.reshape(phi.shape)
+    err = rel_err(got, ref)
     print(f"  max rel err per group {err}", flush=True)
     assert np.all(err <= FP64_TOL)
+    # Per cell, wherever the flux is above 1e-10 of the group's largest: within
+    # 1e-12, or within 20x the reference's own per-cell reproducibility across
+    # OpenMP thread counts where that is larger (diamond-difference cells whose
+    # signed angular fluxes cancel, TW-DD weights at their 1e-8 branch).
+    worst, at = 0.0, None
+    for g in range(ref.shape[0]):
+        big = np.abs(ref[g]) > 1e-10 * np.abs(ref[g]).max()
+        rel = np.abs(got[g][big] - ref[g][big]) / np.abs(ref[g][big])
+        if rel.size and rel.max() > worst:
+            worst = float(rel.max())
+            c = int(np.flatnonzero(big)[rel.argmax()])
+            at = (g, c, float(abs(ref[g][c]) / np.abs(ref[g]).max()))
+    spread = PERCELL_SPREAD[name]
+    print(f"  worst per-cell rel err {worst:.3e} at (group, cell, |phi|/max) {at}; the "
+          f"reference's own spread across thread counts {spread:.3e}", flush=True)
+    assert worst <= max(FP64_TOL, 20.0 * spread)
     if man["log"]["si"] and st["success"]:
         last = {g: k for k, _, g in man["log"]["si"][-p.groups:]}
         assert [last[g] for g in range(p.groups)] == st["si_iterations"]
@@ -141,6 +162,17 @@ def test_slab_five_region_1d():
     assert st["success"] and rst["success"]
     assert st["si_iterations"] == rst["si_iterations"]
     assert np.all(rel_err(phi, ref) <= FP64_TOL)
+    # Per cell, including the shadow of the sigma_t = 50 region: within 1e-12 or
+    # 20x the oracle's own spread between thread counts (summation order).
+    spread = 0.0
+    big = np.abs(ref[0]) > 1e-10 * np.abs(ref[0]).max()
+    for t in (2, 3):
+        alt, _ = ob.solve_fixed_source(p, *fs, nthreads=t)
+        spread = max(spread, np.max(np.abs(alt[0] - ref[0])[big] / np.abs(ref[0])[big]))
+    worst = np.max(np.abs(phi[0] - ref[0])[big] / np.abs(ref[0])[big])
+    print(f"  per-cell worst {worst:.2e} (oracle spread {spread:.2e}, smallest flux "
+          f"{np.abs(ref[0][big]).min() / np.abs(ref[0]).max():.1e} of the largest)", flush=True)
+    assert worst <= max(FP64_TOL, 20.0 * spread)
 
 
 @pytest.mark.parametrize("schedule", [0, 1, 3])
@@ -172,15 +204,22 @@ def test_dataflow_schedule_matches_reference(name, schedule):
     assert np.all(err <= FP64_TOL)
 
 
-@pytest.mark.parametrize("name", ["twdd3d_small", "random_blocks_16c_3g_128d", "twdd2d_small"])
+@pytest.mark.parametrize("name", ["twdd3d_small", "random_blocks_16c_3g_128d", "twdd2d_small",
+                                  "kobayashi1_20c_800d_dd", "random_blocks_24c_3g_32d_dd"])
 def test_fp32_sweep_error_statement(name):
+    """FP32 angular flux and cell solve (FP64 scalar-flux accumulation, sources and
+    residuals) against the reference's own FP64 result: the per-group max and L2
+    relative error is stated and held to 1e-5."""
     p, z, _ = load_golden(name)
-    a, sa = _solve(p, z)
+    ref = z["phi"].reshape(p.groups, -1)
     b, sb = _solve(p, z, precision=capi.FP32_SWEEP)
-    err = rel_err(b, a)
-    l2 = np.sqrt(((b - a) ** 2).sum(axis=1) / np.maximum((a ** 2).sum(axis=1), 1e-300))
-    print(f"FP32-sweep vs FP64 {name}: max rel per group {err}, L2 rel {l2}", flush=True)
-    assert np.all(err < 1e-4)
+    assert sb["success"] == bool(z["ref_success"])
+    err = rel_err(b, ref)
+    l2 = np.sqrt(((b - ref) ** 2).sum(axis=1) / np.maximum((ref ** 2).sum(axis=1), 1e-300))
+    print(f"FP32-sweep vs the FP64 reference, {name}: max rel per group "
+          f"{np.array2string(err, precision=2)}, L2 rel {np.array2string(l2, precision=2)}",
+          flush=True)
+    assert np.all(err < 1e-5) and np.all(l2 < 1e-5)
 
 
 # 2D sweep with one lane per mesh row (schedule 2): rows below, at and above a

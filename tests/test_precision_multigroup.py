@@ -1,0 +1,86 @@
+"""The multigroup full-scatter workload (8 groups, down-scatter to every lower
+group plus up-scatter between the last two, Gauss-Seidel over groups; 256
+directions, TW-DD theta = 0.9) in both precision modes.  The group loop is the
+reference's solveFixedSource (src/TransportSolver3D.cpp:262-402) with
+sourceIteration per group (:458-497).
+
+* 24^3 cells: the FP64 path against the oracle at 1e-12 (max-normalised per
+  group) with identical SI and MG counts, and per cell above 1e-10 of the
+  group's largest flux within 1e-12 or 20x the oracle's own spread between two
+  thread counts (the reference's summation order depends on them); the FP32-sweep mode (FP32 angular flux and cell solve, FP64
+  scalar-flux accumulation, group source and residuals) against the same
+  oracle, with its per-group max and L2 relative error stated and held to 1e-5.
+* 128^3 cells (the full workload): FP32-sweep against FP64, both to a 1e-6
+  residual -- per-group max relative difference <= 1e-5, the same number of
+  multigroup passes, both converged."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from doma_amd import capi, workloads
+from doma_amd.capi import Context
+
+pytestmark = pytest.mark.gpu
+
+ARGS = (1e-6, 1000, 1e-6, 300)
+
+
+def _errors(phi, ref):
+    d = phi - ref
+    mx = np.abs(d).max(axis=1) / np.abs(ref).max(axis=1)
+    l2 = np.linalg.norm(d, axis=1) / np.linalg.norm(ref, axis=1)
+    return mx, l2
+
+
+def test_multigroup_full_24c_fp64_and_fp32_against_oracle():
+    from oracle import binding as ob
+
+    p = workloads.multigroup_full(24, 8, 8, 4)
+    print(f"case {p.name}: oracle solves on the host", flush=True)
+    ref, rst = ob.solve_fixed_source(p.normalized(), *ARGS, nthreads=16)
+    assert rst["success"]
+    # the reference's own per-cell reproducibility: the same solve with another
+    # thread count (another summation order of the scalar flux)
+    ref7, _ = ob.solve_fixed_source(p.normalized(), *ARGS, nthreads=7)
+    big = np.abs(ref) > 1e-10 * np.abs(ref).max(axis=1, keepdims=True)
+    spread = np.max(np.abs(ref7 - ref)[big] / np.abs(ref)[big])
+    with Context(p) as ctx:
+        phi, st = ctx.solve_fixed_source(*ARGS)
+    print(f"  FP64: MG {st['mg_iterations']} (oracle {rst['mg_iterations']}), "
+          f"sweeps {st['sweeps']} (oracle {rst['sweeps']})", flush=True)
+    assert st["success"]
+    assert st["mg_iterations"] == rst["mg_iterations"] and st["sweeps"] == rst["sweeps"]
+    assert st["si_iterations"] == rst["si_iterations"]
+    assert np.all(rel_err(phi, ref) <= 1e-12)
+    worst = np.max(np.abs(phi - ref)[big] / np.abs(ref)[big])
+    print(f"  FP64 per-cell worst {worst:.2e} (reference spread {spread:.2e})", flush=True)
+    assert worst <= max(1e-12, 20.0 * spread)
+    with Context(p, precision=capi.FP32_SWEEP) as ctx:
+        phi32, st32 = ctx.solve_fixed_source(*ARGS)
+    mx, l2 = _errors(phi32, ref)
+    print(f"  FP32-sweep vs FP64 oracle: MG {st32['mg_iterations']}, per group max rel "
+          f"{np.array2string(mx, precision=2)}, L2 rel {np.array2string(l2, precision=2)}",
+          flush=True)
+    assert st32["success"]
+    assert mx.max() <= 1e-5 and l2.max() <= 1e-5
+
+
+def test_multigroup_full_128c_fp32_against_fp64():
+    p = workloads.multigroup_full(128, 8, 8, 4)
+    out = {}
+    for label, prec in (("fp64", capi.FP64), ("fp32", capi.FP32_SWEEP)):
+        print(f"case {p.name} {label}", flush=True)
+        with Context(p, precision=prec) as ctx:
+            ctx.autotune(0)
+            out[label] = ctx.solve_fixed_source(*ARGS)
+        print(f"  {label}: MG {out[label][1]['mg_iterations']}, sweeps {out[label][1]['sweeps']}, "
+              f"{out[label][1]['elapsed_ms'] / 1e3:.2f} s", flush=True)
+    (phi64, st64), (phi32, st32) = out["fp64"], out["fp32"]
+    assert st64["success"] and st32["success"]
+    assert st32["mg_iterations"] == st64["mg_iterations"]
+    mx, l2 = _errors(phi32, phi64)
+    print(f"  FP32-sweep vs FP64: per group max rel {np.array2string(mx, precision=2)}, "
+          f"L2 rel {np.array2string(l2, precision=2)}", flush=True)
+    assert mx.max() <= 1e-5
