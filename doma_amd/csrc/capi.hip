@@ -514,6 +514,7 @@ run_sweep_with(doma_ctx * c, const double * q)
     r.part = c->d_part;
     r.skip = c->si_device ? c->d_si_state : nullptr;
     r.error = c->d_err;
+    r.wait_ticks = (long long)c->wall_clock_khz * 250;  // 250 ms, as the brick waits
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
     int rc = launch_sweep2d_rows(r, c->scheme, c->precision, c->exact, c->stream);

@@ -133,6 +133,7 @@ struct Rows2DArgs
   double * part;           // [4*dirs][N] per-direction w psi
   const int * skip;        // as BrickArgs::skip
   int * error;             // set if a wave-to-wave wait exceeds its bound
+  long long wait_ticks;    // that bound, in device wall-clock ticks
 };
 size_t rows2d_lds(int nx, int ny, size_t rs);
 bool rows2d_fits(int nx, int ny);

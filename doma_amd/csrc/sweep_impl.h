@@ -612,8 +612,15 @@ brick_sweep_kernel(BrickArgs a)
 // Bounded wait for an upwind task's epoch flag (thread 0 only).  Polls with
 // relaxed device-coherent loads; gives up at a wall-clock deadline, or at once
 // when another wait of this sweep has already expired.
+//
+// Visibility of the upwind faces does not rest on release/acquire semantics:
+// the producer's face stores are device-coherent (they bypass the per-XCD L2)
+// and have completed (s_waitcnt) before its flag store, and the consumer's face
+// loads are device-coherent too.  The signal fences below only stop the
+// compiler from moving face accesses across the flag accesses; they emit no
+// instruction.
 __device__ __forceinline__ void
-wait_epoch(const uint32_t * f, uint32_t epoch, int * error, long long wait_ticks)
+wait_epoch_body(const uint32_t * f, uint32_t epoch, int * error, long long wait_ticks)
 {
   if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch)
     return;
@@ -633,6 +640,22 @@ wait_epoch(const uint32_t * f, uint32_t epoch, int * error, long long wait_ticks
       }
     }
   }
+}
+
+__device__ __forceinline__ void
+wait_epoch(const uint32_t * f, uint32_t epoch, int * error, long long wait_ticks)
+{
+  wait_epoch_body(f, epoch, error, wait_ticks);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Publishes a finished task's epoch (thread 0, after every thread's face
+// stores have completed and the work-group has synchronised).
+__device__ __forceinline__ void
+publish_epoch(uint32_t * f, uint32_t epoch)
+{
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(f, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void
@@ -709,8 +732,7 @@ brick_flow_kernel(BrickArgs a)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0)
-      __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      publish_epoch(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch);
   }
 }
 
@@ -780,8 +802,7 @@ brick_pencil_kernel(BrickArgs a)
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (threadIdx.x == 0)
-        __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        publish_epoch(flags + (bk * a.nty + bj) * a.ntx + bi, a.epoch);
     }
   }
 }
@@ -849,8 +870,7 @@ brick_batch_kernel(BatchArgs b)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0)
-      __hip_atomic_store(flags + (bk * a.nty + bj) * a.ntx + bi, b.epoch, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      publish_epoch(flags + (bk * a.nty + bj) * a.ntx + bi, b.epoch);
   }
 }
 

@@ -122,7 +122,7 @@ sweep2d_rows_kernel(Rows2DArgs a)
   // seen: columns of the wave below known to be ready (uniform).  The wave below
   // publishes its count once per group of steps, so the count is read only when
   // this wave runs past the last value it saw.
-  int spins = 0, published = 0, seen = 0;
+  int published = 0, seen = 0;
   R x = R(0), yo = R(0);
   // Eight steps on the inputs in (uq, us) while the next eight cells' inputs
   // load into (lq, ls); two register sets alternate, so no copy waits on them.
@@ -150,15 +150,25 @@ sweep2d_rows_kernel(Rows2DArgs a)
       {
         if (ii0 >= seen)
         {
-          while ((seen = __hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP)) <= ii0)
+          seen = __hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (seen <= ii0)
           {
-            if (++spins > (1 << 26))
+            // Bounded by a wall-clock deadline, as the dataflow waits are; the
+            // sleep leaves issue slots to the producing wave on the same CU.
+            const long long t0 = wall_clock64();
+            int polls = 0;
+            while ((seen = __hip_atomic_load(s_ready + wave - 1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP)) <= ii0)
             {
-              if (lane == 0)
-                __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              seen = a.nx;  // give up: the error flag reports the stall
-              break;
+              __builtin_amdgcn_s_sleep(1);
+              if ((++polls & 15) == 0 && wall_clock64() - t0 > a.wait_ticks)
+              {
+                if (lane == 0)
+                  __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                seen = a.nx;  // give up: the error flag reports the stall
+                break;
+              }
             }
           }
           // Orders every later read of the column after the count that covers it.

@@ -1,8 +1,8 @@
 // unet.h -- the learned source-iteration correction: a 3-level 3D U-Net on
 // BF16 MFMA (FP32 accumulate) that maps the last two SI increments of a group
 // (the SI residual history) plus the group's sigma_t and in-group scattering
-// ratio to an estimate of the remaining error phi_converged - phi, in units of
-// the geometric-series extrapolation of the latest increment.  Host-side
+// ratio to a refinement of the geometric-series extrapolation of the remaining
+// error phi_converged - phi.  Host-side
 // description shared by the C ABI, the kernels (unet.hip) and the trainer.
 #pragma once
 
@@ -87,10 +87,11 @@ struct Net
 int net_setup(Net & n, const float * params, int D, int H, int W);
 void net_free(Net & n);
 
-// corr[c] = s * gain * U-Net(dlast / s, dprev / s, log sigma_t,
-// log 1/(1 - c_gg + 1e-4)) for group g, with s = rms(dlast), rho = rms(dlast) /
+// corr[c] = gain * (dlast[c] + s * U-Net(dlast / s, dprev / s, log sigma_t,
+// log 1/(1 - c_gg + 1e-4))) for group g, with s = rms(dlast), rho = rms(dlast) /
 // rms(dprev) clamped to [0, 0.999] and gain = rho / (1 - rho): the network
-// refines the geometric-series extrapolation gain * dlast of the remaining error.
+// refines the geometric-series extrapolation gain * dlast of the remaining error
+// (a zero network leaves the bare extrapolation).
 // dlast and dprev are the latest two increments of plain source iterations;
 // mat/sigt/sigs as in IterArgs.
 int net_apply(Net & n, const double * dlast, const double * dprev, const int32_t * mat,

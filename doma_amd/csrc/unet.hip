@@ -155,6 +155,109 @@ conv3_kernel(ConvArgs p)
     }
 }
 
+// The same convolution with the input staged in LDS: a work-group computes an
+// output tile of TZ x 8 x 16 voxels (rows of 16 consecutive x voxels, TZ*8/4
+// rows per wave) and first copies the tile's input halo, (TZ+2) x 10 x 18
+// voxels of all cin_p channels, into LDS once -- the decoder's coarse channels
+// gathered from the parent voxel on the way -- so each input value crosses the
+// cache hierarchy once instead of once per tap.  Each wave then issues its
+// row tiles' MFMAs from LDS against one weight fragment per k-step.  Used when
+// a sample's depth Ds is a multiple of TZ (tiles never straddle samples).
+constexpr int kTileX = 16, kTileY = 8;
+
+template <int NT, int TZ>
+__global__ __launch_bounds__(RT) void
+conv3_tile_kernel(ConvArgs p)
+{
+  constexpr int HX = kTileX + 2, HY = kTileY + 2, HZ = TZ + 2, HALO = HX * HY * HZ;
+  constexpr int MT = TZ * kTileY / (RT / 64);  // row tiles per wave
+  extern __shared__ __align__(16) uint16_t tile[];  // [HALO][cin_p]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  const int ntx = (p.W + kTileX - 1) / kTileX, nty = (p.H + kTileY - 1) / kTileY;
+  const int bx = (int)(blockIdx.x % ntx), by = (int)((blockIdx.x / ntx) % nty);
+  const int bz = (int)(blockIdx.x / ((unsigned)ntx * nty));
+  const int x0 = bx * kTileX, y0 = by * kTileY, z0 = bz * TZ;
+  const int zs = z0 - z0 % p.Ds;  // first z of this tile's sample
+  const int cin_p = p.cin_p, cg = cin_p >> 3;
+  const int H2 = p.H >> 1, W2 = p.W >> 1;
+  for (int e = threadIdx.x; e < HALO * cg; e += RT)
+  {
+    const int h = e / cg, c8 = e - h * cg;
+    const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+    const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+    v4u raw = {0u, 0u, 0u, 0u};
+    if (x >= 0 && x < p.W && y >= 0 && y < p.H && z >= zs && z < zs + p.Ds && z < p.D)
+    {
+      const int ci = c8 * 8;
+      if (ci < p.ca)
+        raw = *reinterpret_cast<const v4u *>(p.a + ((size_t)(z * p.H + y) * p.W + x) * p.ca + ci);
+      else
+        raw = *reinterpret_cast<const v4u *>(
+            p.b + ((size_t)((z >> 1) * H2 + (y >> 1)) * W2 + (x >> 1)) * p.cb + (ci - p.ca));
+    }
+    *reinterpret_cast<v4u *>(tile + (size_t)h * cin_p + c8 * 8) = raw;
+  }
+  __syncthreads();
+
+  v4f acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < p.kpad; k0 += 32)
+  {
+    const int k = k0 + kg * 8;
+    const int tap = k / cin_p;
+    const int ci = k - tap * cin_p;
+    v8bf bw[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      bw[nt] = load8(p.w + (size_t)(nt * 16 + row) * p.kpad + k);
+    // K slots past the 27 taps carry zero weights; read a valid slot instead.
+    const int t = tap < 27 ? tap : 13;
+    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets 0..2
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+    {
+      const int r = wave * MT + mt, tz = r / kTileY, ty = r - tz * kTileY;
+      const int h = ((tz + dz) * HY + (ty + dy)) * HX + (row + dx);
+      const v8bf av = load8(tile + (size_t)h * cin_p + ci);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+  {
+    const int r = wave * MT + mt, tz = r / kTileY, ty = r - tz * kTileY;
+    const int z = z0 + tz, y = y0 + ty;
+    if (z >= p.D || y >= p.H)
+      continue;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+    {
+      const int co = nt * 16 + row;
+      if (co >= p.cout)
+        continue;
+      const float bv = p.bias[co];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+      {
+        const int x = x0 + kg * 4 + j;
+        if (x >= p.W)
+          continue;
+        float f = acc[mt][nt][j] + bv;
+        if (p.relu)
+          f = fmaxf(f, 0.f);
+        p.out[((size_t)(z * p.H + y) * p.W + x) * p.out_stride + co] = f2bf(f);
+      }
+    }
+  }
+}
+
 // 2x2x2 average pooling, channels-last.
 __global__ void
 pool_kernel(const uint16_t * in, uint16_t * out, int D, int H, int W, int C)
@@ -281,16 +384,17 @@ head_raw_kernel(const uint16_t * h, int stride, int C, const float * wb, int N, 
     out[c] = head_value(h, stride, C, wb, c);
 }
 
-// Correction: e = s * gain * y, the network's refinement of the geometric-series
-// extrapolation gain * d_k of the remaining error.
+// Correction: e = gain * (d_k + s * y) -- the geometric-series extrapolation
+// gain * d_k of the remaining error plus the network's refinement of it (a zero
+// network leaves the bare extrapolation).
 __global__ void
 head_corr_kernel(const uint16_t * h, int stride, int C, const float * wb, const double * scale,
-                 int N, double * out)
+                 const double * dlast, int N, double * out)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N)
     return;
-  out[c] = scale[0] * scale[1] * (double)head_value(h, stride, C, wb, c);
+  out[c] = scale[1] * (dlast[c] + scale[0] * (double)head_value(h, stride, C, wb, c));
 }
 
 inline unsigned
@@ -299,9 +403,39 @@ nblk(size_t n, int t)
   return (unsigned)((n + t - 1) / t);
 }
 
+// LDS bytes of conv3_tile_kernel's input halo.
+constexpr size_t tile_lds(int tz, int cin_p)
+{
+  return (size_t)(kTileX + 2) * (kTileY + 2) * (tz + 2) * cin_p * 2;
+}
+
+template <int TZ>
+int
+launch_conv_tile(const ConvArgs & p, int cout16, hipStream_t s)
+{
+  const unsigned grid = (unsigned)(((p.W + kTileX - 1) / kTileX) *
+                                   ((p.H + kTileY - 1) / kTileY) * ((p.D + TZ - 1) / TZ));
+  const size_t lds = tile_lds(TZ, p.cin_p);
+  if (cout16 == 16)
+    hipLaunchKernelGGL((conv3_tile_kernel<1, TZ>), dim3(grid), dim3(RT), lds, s, p);
+  else if (cout16 == 32)
+    hipLaunchKernelGGL((conv3_tile_kernel<2, TZ>), dim3(grid), dim3(RT), lds, s, p);
+  else if (cout16 == 64)
+    hipLaunchKernelGGL((conv3_tile_kernel<4, TZ>), dim3(grid), dim3(RT), lds, s, p);
+  else
+    return fail(DOMA_ERR_INVALID, "unsupported layer width");
+  return DOMA_OK;
+}
+
 int
 launch_conv(const ConvArgs & p, int cout16, hipStream_t s)
 {
+  // The LDS-tiled kernel when tiles of 4 (or 2) z-planes do not straddle a
+  // sample and the halo fits 64 KiB of LDS; the direct kernel otherwise.
+  if (p.Ds % 4 == 0 && tile_lds(4, p.cin_p) <= (64u << 10))
+    return launch_conv_tile<4>(p, cout16, s);
+  if (p.Ds % 2 == 0 && tile_lds(2, p.cin_p) <= (64u << 10))
+    return launch_conv_tile<2>(p, cout16, s);
   constexpr int MT = 2;
   const unsigned grid = nblk(p.V, (RT / 64) * 16 * MT);
   if (cout16 == 16)
@@ -507,7 +641,7 @@ net_apply(Net & n, const double * dlast, const double * dprev, const int32_t * m
   if (rc)
     return rc;
   hipLaunchKernelGGL(head_corr_kernel, dim3(nblk(N, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
-                     n.layer[10].bias, n.scale, N, corr);
+                     n.layer[10].bias, n.scale, dlast, N, corr);
   return DOMA_OK;
 }
 

@@ -11,10 +11,12 @@ boundary as one flat FP32 vector, per layer W[cout][tap][cin] (tap = (dz+1)*9 +
 Input channels, per cell, for group g: the last two increments of plain source
 iterations d_k and d_(k-1), both divided by s = rms(d_k) over the grid, log
 sigma_t, and log(1 / (1 - c + 1e-4)) with c the in-group scattering ratio
-sigma_s,gg / sigma_t.  Output y: the remaining error phi_converged - phi_k =
-s * g * y with g = rho / (1 - rho), rho = rms(d_k) / rms(d_(k-1)) clamped to
-0.999 -- the network refines the geometric-series extrapolation g * d_k, which
-is exact when one mode dominates the iteration, into a per-cell estimate.
+sigma_s,gg / sigma_t.  The remaining error is estimated as phi_converged - phi_k
+= g (d_k + s y) with g = rho / (1 - rho), rho = rms(d_k) / rms(d_(k-1)) clamped
+to 0.999: the geometric-series extrapolation g d_k, exact when one mode
+dominates the iteration, plus the network's output y, which learns what the
+extrapolation misses (spatially varying decay where slow modes of different
+rates sit in different regions).  A zero network is the bare extrapolation.
 """
 from __future__ import annotations
 
@@ -124,7 +126,8 @@ def bf16_round(flat):
 
 def collect_samples(batch, problems, max_per_group=None):
     """(features [D,H,W,4], scaled target [D,H,W]) pairs from a solved Batch: the
-    target is (phi_converged - phi_k) / (s g)."""
+    target is what the extrapolation misses, (phi_converged - phi_k) / (s g) -
+    d_k / s."""
     out = []
     for i, p in enumerate(problems):
         n = p.nx
@@ -137,7 +140,7 @@ def collect_samples(batch, problems, max_per_group=None):
                 r, rp, t = batch.get_sample(i, g, k, with_prev=True)
                 f, s, gn = features(r.reshape(n, n, n), rp.reshape(n, n, n), st.reshape(n, n, n),
                                     cgg.reshape(n, n, n))
-                y = t.reshape(n, n, n) / (s * gn if gn > 0 else s)
+                y = t.reshape(n, n, n) / (s * gn if gn > 0 else s) - f[..., 0]
                 out.append((f, y.astype(np.float32)))
     return out
 
@@ -195,41 +198,28 @@ def train(samples, steps=2000, crops=(16, 32), batch_size=8, lr=1e-3, seed=0, de
 
 
 def extrapolation_params() -> np.ndarray:
-    """Hand-set parameters whose output is y = d_k / s (input channel 0) up to
-    BF16 rounding, so that the correction is exactly the geometric-series
-    extrapolation g * d_k: the first layer splits the channel into relu(x) and
-    relu(-x), the full-resolution layers and the decoder's skip path carry both
-    (centre tap), and the head takes their difference; the coarse levels get
-    zero weights.  The ablation that isolates what the trained network adds."""
-    flat = np.zeros(param_count(), dtype=np.float32)
-    off = 0
-    views = []
-    for a, b, co, t in SPEC:
-        ci = a + b
-        w = flat[off:off + co * t * ci].reshape(co, t, ci)
-        views.append(w)
-        off += co * t * ci + co
-    centre = 13  # tap (dz, dy, dx) = (0, 0, 0)
-    views[0][0, centre, 0] = 1.0   # enc1a: +x
-    views[0][1, centre, 0] = -1.0  #        -x
-    for layer in (1, 8, 9):        # enc1b, dec1a (skip channels 0, 1), dec1b
-        views[layer][0, centre, 0] = 1.0
-        views[layer][1, centre, 1] = 1.0
-    views[10][0, 0, 0] = 1.0       # head: relu(x) - relu(-x)
-    views[10][0, 0, 1] = -1.0
-    return flat
+    """The ablation: a zero network, whose correction is the bare geometric-series
+    extrapolation g d_k -- what isolates the trained network's contribution."""
+    return np.zeros(param_count(), dtype=np.float32)
 
 
-def init_params(seed=0):
+def init_params(seed=0, zero_head=False):
     """Fresh parameters without PyTorch: per layer W, b ~ U(-1/sqrt(fan_in),
-    1/sqrt(fan_in)) (the default initialisation of a PyTorch Conv3d)."""
+    1/sqrt(fan_in)) (the default initialisation of a PyTorch Conv3d); with
+    zero_head the 1x1x1 head starts at zero, so training starts from the bare
+    extrapolation."""
     rng = np.random.default_rng(seed)
     out = []
     for a, b, co, t in SPEC:
         fan_in = (a + b) * t
         bound = 1.0 / math.sqrt(fan_in)
-        out.append(rng.uniform(-bound, bound, co * t * (a + b)))
-        out.append(rng.uniform(-bound, bound, co))
+        w = rng.uniform(-bound, bound, co * t * (a + b))
+        bias = rng.uniform(-bound, bound, co)
+        if zero_head and t == 1:
+            w[:] = 0.0
+            bias[:] = 0.0
+        out.append(w)
+        out.append(bias)
     return np.concatenate(out).astype(np.float32)
 
 
@@ -248,7 +238,8 @@ def train_native(samples, steps=3000, crop=32, batch_size=8, lr=1e-3, seed=0, lo
     fit = [samples[i] for i in order[nh:] if samples[i][0].shape[0] >= crop]
     if not fit:
         raise ValueError(f"no training sample is at least {crop} cells wide")
-    flat = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
+    flat = (init_params(seed, zero_head=True) if params is None
+            else np.asarray(params, dtype=np.float32))
     c = crop
     with capi.NetTrainer(flat, crop=c, batch=batch_size) as tr:
         for step in range(steps):
@@ -279,7 +270,8 @@ def training_problems(seed=20261210, small=False):
     8^3-cell blocks) of high-scattering media in the regime the correction
     serves -- sigma_t log-uniform in [0.5, 10], c ~ U[c_lo, 0.995] per (block,
     group) with c_lo in {0.9, 0.95}, 98 % of it in-group, 1 or 2 groups,
-    sources in 5 or 10 % of the blocks -- diamond difference, 128 directions."""
+    sources in 5 or 10 % of the blocks -- diamond difference, 256 directions
+    (8 polar x 4 azimuthal per octant, the learned workload's quadrature)."""
     from . import capi, workloads
     from .capi import MediaSpec
 
@@ -294,7 +286,7 @@ def training_problems(seed=20261210, small=False):
                          c_lo=c_lo, c_hi=0.995, scatter_rule=capi.SCATTER_INGROUP,
                          source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
                          source_fraction=frac, cell_width=1.0, seed=seed * 1000 + b)
-        probs.append(workloads.block_media_problem(n, groups, 4, 4, capi.SCHEME_DD, spec,
+        probs.append(workloads.block_media_problem(n, groups, 8, 4, capi.SCHEME_DD, spec,
                                                    f"train_{b}_{n}c_{groups}g"))
     return probs
 

@@ -257,14 +257,14 @@ int doma_autotune(doma_ctx *ctx, int32_t g, doma_tuning *best, double *best_ms);
 int64_t doma_net_param_count(void);
 
 /* Enable the correction on a 3D context (extents divisible by 4): every `every`
- * source iterations (every >= 2) the network maps the last two increments of
- * the group's plain iterations d_k, d_(k-1) (each divided by rms(d_k)), sigma_t
- * and the in-group scattering ratio to a flux correction s * g * y (BF16 on
- * MFMA, FP32 accumulate), s = rms(d_k), g = rho / (1 - rho), rho = rms(d_k) /
- * rms(d_(k-1)) clamped to 0.999: the network refines the geometric-series
- * extrapolation g * d_k of the remaining error.  The correction is undone if the
- * next residual is not below the one before it.  every = 0 disables it; params
- * may then be NULL. */
+ * source iterations (every >= 2) the remaining error of the group is estimated
+ * as g (d_k + s y): the geometric-series extrapolation g d_k of the latest
+ * plain increment, g = rho / (1 - rho), rho = rms(d_k) / rms(d_(k-1)) clamped
+ * to 0.999, s = rms(d_k), refined by the network's output y (BF16 on MFMA, FP32
+ * accumulate), which reads d_k / s, d_(k-1) / s, sigma_t and the in-group
+ * scattering ratio.  A zero network gives the bare extrapolation.  The
+ * correction is undone if the next residual is not below the one before it.
+ * every = 0 disables it; params may then be NULL. */
 int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t count, int32_t every);
 
 /* Accelerated multigroup iteration (no reference counterpart; off by default):

@@ -42,9 +42,12 @@ def test_fixed_source_matches_reference(name, exact):
     print(f"  max rel err per group {err}", flush=True)
     assert np.all(err <= FP64_TOL)
     # Per cell, wherever the flux is above 1e-10 of the group's largest: within
-    # 1e-12, or within 20x the reference's own per-cell reproducibility across
-    # OpenMP thread counts where that is larger (diamond-difference cells whose
-    # signed angular fluxes cancel, TW-DD weights at their 1e-8 branch).
+    # 1e-12 (the exact arithmetic, in the reference's expression order) or 1e-10
+    # (the fast arithmetic: reordered FMA-contracted weights, whose ulp-level
+    # differences reach ~3e-11 in cells 1e-9 of the largest), or within 20x the
+    # reference's own per-cell reproducibility across OpenMP thread counts where
+    # that is larger (diamond-difference cells whose signed angular fluxes
+    # cancel).
     worst, at = 0.0, None
     for g in range(ref.shape[0]):
         big = np.abs(ref[g]) > 1e-10 * np.abs(ref[g]).max()
@@ -56,7 +59,7 @@ def test_fixed_source_matches_reference(name, exact):
     spread = PERCELL_SPREAD[name]
     print(f"  worst per-cell rel err {worst:.3e} at (group, cell, |phi|/max) {at}; the "
           f"reference's own spread across thread counts {spread:.3e}", flush=True)
-    assert worst <= max(FP64_TOL, 20.0 * spread)
+    assert worst <= max(FP64_TOL if exact else 1e-10, 20.0 * spread)
     if man["log"]["si"] and st["success"]:
         last = {g: k for k, _, g in man["log"]["si"][-p.groups:]}
         assert [last[g] for g in range(p.groups)] == st["si_iterations"]

@@ -80,24 +80,31 @@ def test_net_single_tap_exact():
     assert np.allclose(got, want, rtol=1e-2, atol=1e-2)
 
 
-def _slow_problem(n=16, seed=5):
+def _slow_problem(n=16, seed=5, scheme=capi.SCHEME_TW_DD):
     spec = capi.MediaSpec(n=n, block=8, groups=1, sigt_log=1, sigt_lo=0.5, sigt_hi=10.0,
                           c_lo=0.95, c_hi=0.995, scatter_rule=capi.SCATTER_DOWN1,
                           source_rule=capi.SOURCE_BLOCKS, src_lo=0, src_hi=0,
                           source_fraction=0.2, cell_width=1.0, seed=seed)
-    return workloads.block_media_problem(n, 1, 2, 2, capi.SCHEME_TW_DD, spec, "slow16")
+    return workloads.block_media_problem(n, 1, 2, 2, scheme, spec, "slow16")
 
 
 @pytest.mark.gpu
-def test_zero_network_is_plain_si():
-    p = _slow_problem()
+def test_zero_network_is_the_extrapolation():
+    """A zero network leaves the bare geometric-series extrapolation g d_k: on a
+    slowly converging diamond-difference problem it is accepted, cuts the
+    sweeps, and the iteration ends at the plain fixed point (DD source
+    iteration is linear)."""
+    p = _slow_problem(scheme=capi.SCHEME_DD)
     with doma_amd.Context(p) as ctx:
-        phi0, st0 = ctx.solve_fixed_source(1e-8, 3000, 1e-8, 10)
+        phi0, st0 = ctx.solve_fixed_source(1e-10, 5000, 1e-10, 10)
         ctx.set_learned_correction(np.zeros(unet.param_count(), np.float32), every=5)
-        phi1, st1 = ctx.solve_fixed_source(1e-8, 3000, 1e-8, 10)
-    assert st0["si_iterations"] == st1["si_iterations"]
-    assert np.array_equal(phi0, phi1)
-    assert st1["corrections_accepted"] + st1["corrections_rejected"] > 0
+        phi1, st1 = ctx.solve_fixed_source(1e-10, 5000, 1e-10, 10)
+    print("zero network: sweeps", st0["sweeps"], "->", st1["sweeps"], "corrections",
+          st1["corrections_accepted"], "+/", st1["corrections_rejected"], "-", flush=True)
+    assert st0["success"] and st1["success"]
+    assert st1["corrections_accepted"] > 0
+    assert st1["sweeps"] < st0["sweeps"]
+    assert rel_err(phi1, phi0).max() < 1e-7
 
 
 @pytest.mark.gpu
@@ -214,15 +221,11 @@ def test_native_training_reduces_loss():
     assert np.isfinite(p).all() and not np.array_equal(p, flat)
 
 
-def test_extrapolation_params_pass_the_increment_through():
-    """The hand-set ablation network outputs its first input channel exactly (in
-    FP32), so its correction is the plain geometric-series extrapolation."""
-    torch = pytest.importorskip("torch")
-    m = unet.unflatten(unet.make_model(), unet.extrapolation_params())
-    x = torch.randn(2, unet.CIN, 8, 12, 16, generator=torch.Generator().manual_seed(13))
-    with torch.no_grad():
-        y = m(x)
-    assert torch.equal(y[:, 0], x[:, 0])
+def test_extrapolation_ablation_is_the_zero_network():
+    assert unet.extrapolation_params().shape == (unet.param_count(),)
+    assert not unet.extrapolation_params().any()
+    head = unet.init_params(seed=3, zero_head=True)[-(unet.C1 + 1):]
+    assert not head.any()
 
 
 def test_gain_and_features_host():
