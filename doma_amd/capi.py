@@ -32,6 +32,7 @@ SCATTER_FULL = 1
 SCATTER_INGROUP = 2
 STOP_RESIDUAL = 0
 NET_CIN = 4  # input channels of the correction network
+NET_NOALLOC_MAX_VOXELS = 32768
 STOP_ERROR_ESTIMATE = 1
 SOURCE_CENTRAL = 0
 SOURCE_BLOCKS = 1
@@ -142,6 +143,8 @@ SIGNATURES = [
                                               C.c_int32]),
     ("doma_set_warm_start", C.c_int, [C.c_void_p, C.c_int32]),
     ("doma_set_stopping_rule", C.c_int, [C.c_void_p, C.c_int32]),
+    ("doma_net_forward_noalloc", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                           C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("doma_net_forward", C.c_int, [C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32,
                                    C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("doma_net_trainer_create", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_float), C.c_int64,

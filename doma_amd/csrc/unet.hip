@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "internal.h"
@@ -1403,5 +1404,222 @@ trainer_params(Trainer * t, float * params)
   return DOMA_OK;
 }
 
+// ------------------------------------------------------------------------- //
+// Forward pass without allocation                                            //
+// ------------------------------------------------------------------------- //
+//
+// doma_net_forward_noalloc runs the same layers as doma_net_forward on device
+// buffers the caller owns, with the packed BF16 weights and every activation in
+// one file-scope device array (`scratch`), sized for grids of up to
+// DOMA_NET_NOALLOC_MAX_VOXELS voxels.  Host code never uses the array's name as
+// an address: its device address is looked up once per device with
+// hipGetSymbolAddress and kept in noalloc_base[].
+namespace
+{
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Byte offsets of the scratch regions (weights first, then activations sized for
+// the largest grid); every region starts on a 256-byte boundary.
+struct NoAllocLayout
+{
+  size_t w[kLayers] = {};  // packed weights of the 3x3x3 layers
+  size_t feat = 0, e1a = 0, e1b = 0, p1 = 0, e2a = 0, e2b = 0, p2 = 0, ba = 0, bb = 0, d2a = 0,
+         d2b = 0, d1a = 0, d1b = 0, act_begin = 0, bytes = 0;
+};
+
+constexpr int64_t layer_kpad(const LayerSpec & sp)
+{
+  return (27 * (pad8(sp.cin_a) + (sp.cin_b ? pad8(sp.cin_b) : 0)) + 31) / 32 * 32;
+}
+
+constexpr NoAllocLayout noalloc_layout()
+{
+  NoAllocLayout L{};
+  size_t off = 0;
+  for (int l = 0; l < kLayers; ++l)
+  {
+    const LayerSpec & sp = kSpec[l];
+    if (sp.taps != 27)
+      continue;
+    L.w[l] = off;
+    off = align_up(off + (size_t)((sp.cout + 15) / 16 * 16) * layer_kpad(sp) * 2);
+  }
+  const size_t V = DOMA_NET_NOALLOC_MAX_VOXELS, V2 = V / 8, V4 = V / 64;
+  L.act_begin = off;
+  auto take = [&off](size_t elems) {
+    const size_t at = off;
+    off = align_up(off + elems * 2);
+    return at;
+  };
+  L.feat = take(V * 8);
+  L.e1a = take(V * pad8(C1));
+  L.e1b = take(V * pad8(C1));
+  L.p1 = take(V2 * pad8(C1));
+  L.e2a = take(V2 * pad8(C2));
+  L.e2b = take(V2 * pad8(C2));
+  L.p2 = take(V4 * pad8(C2));
+  L.ba = take(V4 * pad8(C3));
+  L.bb = take(V4 * pad8(C3));
+  L.d2a = take(V2 * pad8(C2));
+  L.d2b = take(V2 * pad8(C2));
+  L.d1a = take(V * pad8(C1));
+  L.d1b = take(V * pad8(C1));
+  L.bytes = off;
+  return L;
+}
+constexpr NoAllocLayout kNoAlloc = noalloc_layout();
+static_assert(kNoAlloc.bytes <= ((size_t)8 << 20), "the forward-pass scratch stays under 8 MiB");
+
+// The scratch array itself (one copy per device, alive until the process ends).
+static __device__ __align__(256) unsigned char scratch[kNoAlloc.bytes];
+
+constexpr int kMaxDevices = 64;
+unsigned char * noalloc_base[kMaxDevices] = {};
+std::mutex noalloc_mu;
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, with that device current).
+int noalloc_scratch(unsigned char ** out)
+{
+  int dev = 0;
+  DOMA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DOMA_ERR_DEVICE, "doma_net_forward_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_base[dev])
+  {
+    void * p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DOMA_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                       hipGetErrorString(e));
+    noalloc_base[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = noalloc_base[dev];
+  return DOMA_OK;
+}
+
+// W[co][tap][cin] (FP32, flat parameter layout) -> BF16 [cout16][kpad] with
+// K = tap * cin_p + slot, zero in the padding slots (the layout net_setup packs).
+__global__ void
+pack_layer_kernel(const float * Wt, int cout, int cin_a, int cin_b, int cin_a_p, int cin_p,
+                  int kpad, int n, uint16_t * w)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int co = i / kpad, k = i - co * kpad;
+  const int tap = k / cin_p, slot = k - tap * cin_p;
+  int ci = -1;
+  if (slot < cin_a_p)
+    ci = slot < cin_a ? slot : -1;
+  else if (slot - cin_a_p < cin_b)
+    ci = cin_a + (slot - cin_a_p);
+  const int cin = cin_a + cin_b;
+  w[i] = (co < cout && tap < 27 && ci >= 0) ? f2bf(Wt[((size_t)co * 27 + tap) * cin + ci]) : 0;
+}
+
+__global__ void
+fill_u32_kernel(uint32_t * p, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0u;
+}
+} // namespace
+
+int
+net_forward_noalloc(const float * d_params, int D, int H, int W, const float * d_features,
+                    float * d_out, hipStream_t s)
+{
+  const size_t V = (size_t)D * H * W;
+  if (D % 4 || H % 4 || W % 4 || D < 4 || H < 4 || W < 4)
+    return fail(DOMA_ERR_INVALID, "doma_net_forward_noalloc: every extent must be a positive "
+                                  "multiple of 4");
+  if (V > DOMA_NET_NOALLOC_MAX_VOXELS)
+    return fail(DOMA_ERR_INVALID, "doma_net_forward_noalloc: " + std::to_string(V) +
+                                      " voxels exceed the scratch limit of " +
+                                      std::to_string(DOMA_NET_NOALLOC_MAX_VOXELS));
+  unsigned char * base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc)
+    return rc;
+  // Activations: padding channels are read (with zero weights) and must hold
+  // zeros, and the array keeps whatever the previous call left: clear the
+  // activation regions first (a fill kernel, no allocation).
+  const size_t words = (kNoAlloc.bytes - kNoAlloc.act_begin) / 4;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(words, 256)), dim3(256), 0, s,
+                     reinterpret_cast<uint32_t *>(base + kNoAlloc.act_begin), words);
+  Net n;
+  n.D = D;
+  n.H = H;
+  n.W = W;
+  int64_t off = 0;
+  for (int l = 0; l < kLayers; ++l)
+  {
+    const LayerSpec & sp = kSpec[l];
+    PackedLayer & L = n.layer[l];
+    const int cin = sp.cin_a + sp.cin_b;
+    L.cout = sp.cout;
+    L.relu = sp.relu;
+    L.taps = sp.taps;
+    if (sp.taps == 1)
+    {
+      // Head: its W[0][0][ci] and bias are contiguous in the flat parameters.
+      L.bias = const_cast<float *>(d_params + off);
+      L.cin_a_p = cin;
+    }
+    else
+    {
+      L.cin_a_p = pad8(sp.cin_a);
+      L.cin_b_p = sp.cin_b ? pad8(sp.cin_b) : 0;
+      L.cout16 = (sp.cout + 15) / 16 * 16;
+      L.kpad = (int)layer_kpad(sp);
+      L.w = reinterpret_cast<uint16_t *>(base + kNoAlloc.w[l]);
+      L.bias = const_cast<float *>(d_params + off + (int64_t)sp.cout * 27 * cin);
+      const int nw = L.cout16 * L.kpad;
+      hipLaunchKernelGGL(pack_layer_kernel, dim3(nblk(nw, 256)), dim3(256), 0, s, d_params + off,
+                         sp.cout, sp.cin_a, sp.cin_b, L.cin_a_p, L.cin_a_p + L.cin_b_p, L.kpad,
+                         nw, L.w);
+    }
+    off += layer_params(sp);
+  }
+  auto act = [base](size_t o) { return reinterpret_cast<uint16_t *>(base + o); };
+  n.feat = act(kNoAlloc.feat);
+  n.e1a = act(kNoAlloc.e1a);
+  n.e1b = act(kNoAlloc.e1b);
+  n.p1 = act(kNoAlloc.p1);
+  n.e2a = act(kNoAlloc.e2a);
+  n.e2b = act(kNoAlloc.e2b);
+  n.p2 = act(kNoAlloc.p2);
+  n.ba = act(kNoAlloc.ba);
+  n.bb = act(kNoAlloc.bb);
+  n.d2a = act(kNoAlloc.d2a);
+  n.d2b = act(kNoAlloc.d2b);
+  n.d1a = act(kNoAlloc.d1a);
+  n.d1b = act(kNoAlloc.d1b);
+  n.ready = true;
+  hipLaunchKernelGGL(features_f32_kernel, dim3(nblk(V, 256)), dim3(256), 0, s, d_features,
+                     (int)V, n.feat);
+  if ((rc = body(n, s)))
+    return rc;
+  hipLaunchKernelGGL(head_raw_kernel, dim3(nblk(V, 256)), dim3(256), 0, s, n.d1b, pad8(C1), C1,
+                     n.layer[10].bias, (int)V, d_out);
+  DOMA_HIP(hipGetLastError());
+  return DOMA_OK;
+}
+
 } // namespace net
 } // namespace doma
+
+extern "C" int
+doma_net_forward_noalloc(const float * d_params, int64_t count, int32_t D, int32_t H, int32_t W,
+                         const float * d_features, float * d_out, void * stream)
+{
+  using namespace doma;
+  if (!d_params || !d_features || !d_out || count != net::total_params())
+    return fail(DOMA_ERR_INVALID, "doma_net_forward_noalloc: bad arguments");
+  return net::net_forward_noalloc(d_params, D, H, W, d_features, d_out,
+                                  static_cast<hipStream_t>(stream));
+}

@@ -273,6 +273,23 @@ int doma_set_learned_correction(doma_ctx *ctx, const float *params, int64_t coun
  * TransportSolver3D.cpp:399-400).  2D/3D. */
 int doma_set_warm_start(doma_ctx *ctx, int32_t enable);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): doma_net_forward on buffers the caller owns, allocating
+ * nothing.  d_params (the flat FP32 parameters, `count` values), d_features
+ * ([V][4] FP32, V = D*H*W, x fastest) and d_out ([V] FP32) are device pointers;
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the null stream) and
+ * the call returns without waiting.  The packed BF16 weights and every
+ * activation live in one static device array of the library (under 8 MiB per
+ * device, held until the process ends), so D, H, W must be multiples of 4 with
+ * D*H*W <= DOMA_NET_NOALLOC_MAX_VOXELS (32^3), else DOMA_ERR_INVALID.  Calls
+ * share that array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller; calls in order on one
+ * stream are safe (each clears what it needs first). */
+#define DOMA_NET_NOALLOC_MAX_VOXELS 32768
+int doma_net_forward_noalloc(const float *d_params, int64_t count, int32_t D, int32_t H,
+                             int32_t W, const float *d_features, float *d_out, void *stream);
+
 /* Stopping rule of source iteration (no reference counterpart for rule 1).
  * DOMA_STOP_RESIDUAL (default): the reference's rule, stop once the relative
  * increment r_k = ||dphi_k|| / ||phi_k|| <= tol (TransportSolver3D.cpp:458-497).
