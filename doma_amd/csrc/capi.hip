@@ -704,16 +704,19 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
   const bool err_rule = c->stop_rule == DOMA_STOP_ERROR_ESTIMATE;
   if (!learned && !err_rule && c->si_chunked)
     return si_loop_device(c, g, tol, max_it, its, res, conv, sweep_ms, sweeps, first_corr);
-  // Error-estimate rule: rho from ratios of consecutive plain iterations.
+  // Error-estimate rule: rho is the largest ratio of consecutive plain
+  // iterations seen so far in this solve for group g -- the ratios approach the
+  // spectral radius from below, and after a correction the slow mode's share is
+  // reduced for a while, so a recent ratio alone can understate it.
   if ((int)c->rho_est.size() < c->G)
     c->rho_est.assign(c->G, -1.0);
-  double rho_prev = c->rho_est[g], r_plain = -1.0;
+  double r_plain = -1.0;
   auto not_done = [&](double r_) {
     if (!err_rule)
       return tol < r_;
     if (r_ == 0.0)
       return false;
-    const double rho = std::min(std::max(c->rho_est[g], rho_prev), 0.9999);
+    const double rho = std::min(c->rho_est[g], 0.9999);
     return rho < 0.0 || tol < r_ * rho / (1.0 - rho);
   };
   IterArgs a = iter_args(c, g);
@@ -763,10 +766,7 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       // follows a correction nor a warm start's first increment).
       const bool plain = !pending && !(it == 1 && first_corr);
       if (plain && r_plain > 0.0)
-      {
-        rho_prev = c->rho_est[g];
-        c->rho_est[g] = rn / r_plain;
-      }
+        c->rho_est[g] = std::max(c->rho_est[g], rn / r_plain);
       r_plain = plain ? rn : -1.0;
     }
     if (!learned)

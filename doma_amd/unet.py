@@ -203,6 +203,15 @@ def extrapolation_params() -> np.ndarray:
     return np.zeros(param_count(), dtype=np.float32)
 
 
+def scale_refinement(params, beta: float) -> np.ndarray:
+    """Scale the network's output (the 1x1x1 head) by beta: the correction becomes
+    g (d_k + beta s y).  beta is chosen on held-out solves; 0 is the bare
+    extrapolation."""
+    flat = np.array(params, dtype=np.float32, copy=True)
+    flat[param_count() - (C1 + 1):] *= beta
+    return flat
+
+
 def init_params(seed=0, zero_head=False):
     """Fresh parameters without PyTorch: per layer W, b ~ U(-1/sqrt(fan_in),
     1/sqrt(fan_in)) (the default initialisation of a PyTorch Conv3d); with
@@ -351,6 +360,8 @@ def main(argv=None):
     ap.add_argument("--sample-cap", type=int, default=16)
     ap.add_argument("--torch", action="store_true",
                     help="train the PyTorch twin instead of the library's kernels")
+    ap.add_argument("--beta", type=float, default=1.0,
+                    help="scale of the trained refinement in the saved weights")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -374,8 +385,9 @@ def main(argv=None):
         flat, held_err = train(samples, steps=args.steps, log=log)
     else:
         flat, held_err = train_native(samples, steps=args.steps, log=log)
-    np.save(out, flat)
-    log(f"held-out relative L2 error of the predicted correction: {held_err:.3f}")
+    np.save(out, scale_refinement(flat, args.beta))
+    log(f"held-out relative L2 error of the predicted refinement: {held_err:.3f}"
+        + (f"; saved with the refinement scaled by {args.beta}" if args.beta != 1.0 else ""))
     log(f"saved {out} ({flat.size} parameters) after {time.time() - t0:.0f} s")
 
 

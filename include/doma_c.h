@@ -294,11 +294,12 @@ int doma_net_forward_noalloc(const float *d_params, int64_t count, int32_t D, in
  * DOMA_STOP_RESIDUAL (default): the reference's rule, stop once the relative
  * increment r_k = ||dphi_k|| / ||phi_k|| <= tol (TransportSolver3D.cpp:458-497).
  * DOMA_STOP_ERROR_ESTIMATE: stop once r_k * rho / (1 - rho) <= tol, where rho is
- * the larger of the last two ratios r_k / r_(k-1) of consecutive plain source
- * iterations (no correction in between) -- the geometric-series estimate of the
- * remaining error phi_converged - phi_k relative to phi_k; the group keeps its
- * estimate across multigroup passes.  Rule 1 runs the iteration with one host
- * round trip per sweep. */
+ * the largest ratio r_k / r_(k-1) of consecutive plain source iterations (no
+ * correction in between) seen so far in the solve for the group (clamped to
+ * 0.9999) -- the geometric-series estimate of the remaining error
+ * phi_converged - phi_k relative to phi_k; the group keeps its estimate across
+ * multigroup passes.  Rule 1 runs the iteration with one host round trip per
+ * sweep. */
 #define DOMA_STOP_RESIDUAL 0
 #define DOMA_STOP_ERROR_ESTIMATE 1
 int doma_set_stopping_rule(doma_ctx *ctx, int32_t rule);
