@@ -134,9 +134,17 @@ def sweeps_only(args):
 
     if args.workload == "multigroup_full":
         p = workloads.multigroup_full(args.n, 8, 8, 4)
+    elif args.workload == "high_scatter":
+        p = workloads.high_scatter(args.n, 8, 8, 4)
     else:
         p = workloads.random_blocks(args.n, 3, 4, 4)
     ctx = Context(p, precision=FP32_SWEEP if args.fp32 else FP64)
+    if args.autotune:  # report the tuned shape and schedule (nothing timed)
+        t, ms = ctx.autotune(0)
+        print(json.dumps({"workload": p.name, "fp32": bool(args.fp32), "tuning": t,
+                          "ms_per_sweep": round(ms, 4)}), flush=True)
+        ctx.close()
+        return
     ctx.set_tuning(shape=args.shape, schedule=args.schedule, exact_arith=int(args.exact))
     q = np.random.default_rng(0).uniform(0.0, 1.0, p.N) / (4 * np.pi)
     ctx.sweep(0, q)
@@ -362,6 +370,24 @@ def learned_mode(args):
           flush=True)
 
 
+def net_mode(args):
+    """Profiling mode: the correction network's forward pass on an n^3 grid of
+    random 4-channel features, repeated --steps times (random-init weights; the
+    kernels and their shapes do not depend on the weight values)."""
+    import numpy as np
+
+    from doma_amd import capi, unet
+
+    w = unet.init_params(seed=1)
+    f = np.random.default_rng(0).standard_normal((args.n, args.n, args.n, unet.CIN)).astype(np.float32)
+    for r in range(args.steps):
+        t = time.perf_counter()
+        y = capi.net_forward(w, f)
+        log(f"net forward {args.n}^3 rep {r}: {time.perf_counter() - t:.3f} s")
+    print(json.dumps({"net_forward_n": args.n, "reps": args.steps,
+                      "out_mean_abs": float(np.abs(y).mean())}), flush=True)
+
+
 def copy_calibration(nbytes=256 << 20, reps=3):
     """A plain device-to-device copy of known size, for scaling HBM counters."""
     import torch
@@ -387,9 +413,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-problem batched solve")
     ap.add_argument("--exact", action="store_true", help="reference expression order")
     ap.add_argument("--mode", choices=["solve", "sweeps", "copy", "batch", "learned", "precision",
-                                       "deck"], default="solve")
+                                       "deck", "net"], default="solve")
     ap.add_argument("--deck", default=None, help="input deck (deck mode)")
-    ap.add_argument("--autotune", action="store_true", help="deck mode: autotune the sweep")
+    ap.add_argument("--autotune", action="store_true",
+                    help="deck mode: autotune the sweep; sweeps mode: print the tuned shape")
     ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
     ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
     ap.add_argument("--weights", default=None, help="learned mode: correction weights (.npy)")
@@ -418,6 +445,8 @@ def main():
         return precision_mode(args)
     if args.mode == "deck":
         return deck_mode(args)
+    if args.mode == "net":
+        return net_mode(args)
 
     from doma_amd import workloads
     from doma_amd.capi import FP64, Context
