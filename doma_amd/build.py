@@ -35,6 +35,14 @@ COMMON = [
 ]
 
 
+# Per-source flags.  The network's convolution kernels keep their MFMA
+# accumulators in ordinary VGPRs: with AGPR accumulators the register allocator
+# shuffles them through v_accvgpr moves inside the k-loop.
+EXTRA = {
+    "unet.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def _sources():
     # The command-line driver (doma_main.cpp) is linked separately.
     srcs = list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")) + list(FRONTEND.glob("*.cpp"))
@@ -50,7 +58,7 @@ def _compile(src: Path, force: bool) -> Path:
     newest_dep = max([src.stat().st_mtime] + [h.stat().st_mtime for h in _headers()])
     if not force and obj.exists() and obj.stat().st_mtime >= newest_dep:
         return obj
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-x", "hip", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <type_traits>
 
 #include "internal.h"
 #include "unet.h"
@@ -31,6 +32,9 @@ namespace
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 
 constexpr int RT = 256;
 
@@ -73,6 +77,7 @@ struct ConvArgs
   uint16_t * out;      // [V][out_stride]
   int out_stride;
   int relu;
+  int four;            // only input channels 0..3 are non-zero (cin_p = 8, no coarse input)
 };
 
 template <int NT, int MT>
@@ -156,23 +161,43 @@ conv3_kernel(ConvArgs p)
     }
 }
 
-// The same convolution with the input staged in LDS: a work-group computes an
-// output tile of TZ x 8 x 16 voxels (rows of 16 consecutive x voxels, TZ*8/4
-// rows per wave) and first copies the tile's input halo, (TZ+2) x 10 x 18
-// voxels of all cin_p channels, into LDS once -- the decoder's coarse channels
-// gathered from the parent voxel on the way -- so each input value crosses the
-// cache hierarchy once instead of once per tap.  Each wave then issues its
-// row tiles' MFMAs from LDS against one weight fragment per k-step.  Used when
-// a sample's depth Ds is a multiple of TZ (tiles never straddle samples).
+// The same convolution with the input staged in LDS, CG = 8 channels at a time:
+// a work-group computes an output tile of TZ x 8 x 16 voxels (rows of 16
+// consecutive x voxels, TZ*8/4 rows per wave).  For each group of 8 input
+// channels it copies the tile's input halo, (TZ+2) x 10 x 18 voxels, and the
+// group's weight fragments into LDS (the decoder's coarse channels gathered from
+// the parent voxel on the way), so each input value crosses the cache hierarchy
+// once instead of once per tap, and then runs 7 k-steps of 4 taps x 8 channels
+// with both operands read from LDS.  The next group's halo and weights are
+// loaded into registers while the current group is multiplied.  The MFMA
+// computes the transposed tile, out^T = W^T x in^T: the A operand is one weight
+// fragment per k-step (16 output channels x 4 taps x 8 channels) and the B
+// operand is the voxel row's 8 channels at one tap, so each lane ends up holding
+// 4 consecutive output channels of one voxel and stores them as one 8-byte
+// write.  A layer with 4 input channels (CG = 4) packs 8 taps x 4 channels per
+// k-step instead (4 k-steps, no zero half): each lane's K slice is two taps of
+// its voxel.  Used when a sample's depth Ds is a multiple of TZ (tiles never
+// straddle samples) and the output channels come in fours.
 constexpr int kTileX = 16, kTileY = 8;
 
-template <int NT, int TZ>
+template <int NT, int TZ, int CG>
 __global__ __launch_bounds__(RT) void
 conv3_tile_kernel(ConvArgs p)
 {
+  static_assert(CG == 8 || CG == 4, "channels per LDS group");
+  using SV = std::conditional_t<CG == 8, v4u, v2u>;  // one voxel's group of channels
   constexpr int HX = kTileX + 2, HY = kTileY + 2, HZ = TZ + 2, HALO = HX * HY * HZ;
   constexpr int MT = TZ * kTileY / (RT / 64);  // row tiles per wave
-  extern __shared__ __align__(16) uint16_t tile[];  // [HALO][cin_p]
+  constexpr int SI = (HALO + RT - 1) / RT;     // halo voxels staged per thread
+  constexpr int NTAP = CG == 8 ? 28 : 32;      // tap slots (27 real, the rest zero weight)
+  constexpr int TPS = 32 / CG;                 // taps per k-step
+  constexpr int WS = NTAP * 16 * NT;           // weight fragments (CG channels) per group
+  constexpr int WI = (WS + RT - 1) / RT;       // weight fragments staged per thread
+  // LDS: the halo [HALO][CG]; the weights, CG = 8: [28 taps][16 NT rows][8],
+  // CG = 4: [16 NT rows][32 taps][4] (a lane's two taps are adjacent).
+  __shared__ __align__(16) uint16_t tile[HALO * CG];
+  __shared__ __align__(16) uint16_t wtile[WS * CG];
+  __shared__ int tapoff[32];  // halo offset of each tap (zero-weight slots: any voxel)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, kg = lane >> 4;
   const int ntx = (p.W + kTileX - 1) / kTileX, nty = (p.H + kTileY - 1) / kTileY;
@@ -180,104 +205,187 @@ conv3_tile_kernel(ConvArgs p)
   const int bz = (int)(blockIdx.x / ((unsigned)ntx * nty));
   const int x0 = bx * kTileX, y0 = by * kTileY, z0 = bz * TZ;
   const int zs = z0 - z0 % p.Ds;  // first z of this tile's sample
-  const int cin_p = p.cin_p, cg = cin_p >> 3;
   const int H2 = p.H >> 1, W2 = p.W >> 1;
-  for (int e = threadIdx.x; e < HALO * cg; e += RT)
+
+  // Source voxels of this thread's halo slots (-1: zero padding or no slot).
+  int va[SI], vb[SI];
+#pragma unroll
+  for (int i = 0; i < SI; ++i)
   {
-    const int h = e / cg, c8 = e - h * cg;
+    const int h = threadIdx.x + i * RT;
     const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
     const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-    v4u raw = {0u, 0u, 0u, 0u};
-    if (x >= 0 && x < p.W && y >= 0 && y < p.H && z >= zs && z < zs + p.Ds && z < p.D)
-    {
-      const int ci = c8 * 8;
-      if (ci < p.ca)
-        raw = *reinterpret_cast<const v4u *>(p.a + ((size_t)(z * p.H + y) * p.W + x) * p.ca + ci);
-      else
-        raw = *reinterpret_cast<const v4u *>(
-            p.b + ((size_t)((z >> 1) * H2 + (y >> 1)) * W2 + (x >> 1)) * p.cb + (ci - p.ca));
-    }
-    *reinterpret_cast<v4u *>(tile + (size_t)h * cin_p + c8 * 8) = raw;
+    const bool in = h < HALO && x >= 0 && x < p.W && y >= 0 && y < p.H && z >= zs &&
+                    z < zs + p.Ds && z < p.D;
+    va[i] = in ? (z * p.H + y) * p.W + x : -1;
+    vb[i] = in ? ((z >> 1) * H2 + (y >> 1)) * W2 + (x >> 1) : -1;
   }
-  __syncthreads();
+  // A channel group's halo and weight fragments, loaded into registers.
+  SV stage[SI], wstage[WI];
+  auto fetch = [&](int c0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < SI; ++i)
+    {
+      SV raw = {};
+      if (va[i] >= 0)
+      {
+        if (c0 < p.ca)
+          raw = *reinterpret_cast<const SV *>(p.a + (size_t)va[i] * p.ca + c0);
+        else
+          raw = *reinterpret_cast<const SV *>(p.b + (size_t)vb[i] * p.cb + (c0 - p.ca));
+      }
+      stage[i] = raw;
+    }
+#pragma unroll
+    for (int i = 0; i < WI; ++i)
+    {
+      const int e = threadIdx.x + i * RT;
+      int tap, co;
+      if constexpr (CG == 8)
+      {
+        tap = e / (16 * NT);
+        co = e - tap * (16 * NT);
+      }
+      else
+      {
+        co = e / NTAP;
+        tap = e - co * NTAP;
+      }
+      SV wr = {};
+      if (e < WS && tap < 27)
+        wr = *reinterpret_cast<const SV *>(p.w + (size_t)co * p.kpad + tap * p.cin_p + c0);
+      wstage[i] = wr;
+    }
+  };
 
+  if (threadIdx.x < 32)
+  {
+    const int t = threadIdx.x < 27 ? threadIdx.x : 13;
+    tapoff[threadIdx.x] = ((t / 9) * HY + (t / 3) % 3) * HX + t % 3;
+  }
   v4f acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
       acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < p.kpad; k0 += 32)
+
+  const int cend = CG == 8 ? p.cin_p : 1;  // CG = 4: the one group of channels 0..3
+  fetch(0);
+  for (int c0 = 0; c0 < cend; c0 += 8)
   {
-    const int k = k0 + kg * 8;
-    const int tap = k / cin_p;
-    const int ci = k - tap * cin_p;
-    v8bf bw[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-      bw[nt] = load8(p.w + (size_t)(nt * 16 + row) * p.kpad + k);
-    // K slots past the 27 taps carry zero weights; read a valid slot instead.
-    const int t = tap < 27 ? tap : 13;
-    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets 0..2
+    for (int i = 0; i < SI; ++i)
+      if (threadIdx.x + i * RT < HALO)
+        *reinterpret_cast<SV *>(tile + (size_t)(threadIdx.x + i * RT) * CG) = stage[i];
+#pragma unroll
+    for (int i = 0; i < WI; ++i)
+      if (threadIdx.x + i * RT < WS)
+        *reinterpret_cast<SV *>(wtile + (size_t)(threadIdx.x + i * RT) * CG) = wstage[i];
+    __syncthreads();
+    // The next group's global loads stay in flight over this group's MFMAs.
+    if (c0 + 8 < cend)
+      fetch(c0 + 8);
+    // NTAP / TPS k-steps, operands from LDS only.
+#pragma unroll 1
+    for (int t0 = 0; t0 < NTAP; t0 += TPS)
+    {
+      v8bf bw[NT];
+      int hoff0, hoff1 = 0;
+      if constexpr (CG == 8)
+      {
+        const int tap = t0 + kg;
+        hoff0 = tapoff[tap];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          bw[nt] = load8(wtile + (size_t)(tap * 16 * NT + nt * 16 + row) * 8);
+      }
+      else
+      {
+        const int tap = t0 + 2 * kg;
+        hoff0 = tapoff[tap];
+        hoff1 = tapoff[tap + 1];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          bw[nt] = load8(wtile + (size_t)((nt * 16 + row) * NTAP + tap) * 4);
+      }
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+      {
+        const int r = wave * MT + mt, tz = r / kTileY, ty = r - tz * kTileY;
+        const int hb = (tz * HY + ty) * HX + row;
+        v8bf av;
+        if constexpr (CG == 8)
+          av = load8(tile + (size_t)(hb + hoff0) * 8);
+        else
+        {
+          const v2u lo = *reinterpret_cast<const v2u *>(tile + (size_t)(hb + hoff0) * 4);
+          const v2u hi = *reinterpret_cast<const v2u *>(tile + (size_t)(hb + hoff1) * 4);
+          av = __builtin_bit_cast(v8bf, v4u{lo[0], lo[1], hi[0], hi[1]});
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[nt], av, acc[mt][nt], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // the next group's staging overwrites the halo
+  }
+  // D layout: lane holds output channels nt*16 + 4*kg + j (j = 0..3) of voxel
+  // x0 + row of its row tile.
+  const int x = x0 + row;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+  {
+    const int co = nt * 16 + 4 * kg;
+    if (co >= p.cout)
+      continue;
+    const v4f bv = {p.bias[co], p.bias[co + 1], p.bias[co + 2], p.bias[co + 3]};
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
     {
       const int r = wave * MT + mt, tz = r / kTileY, ty = r - tz * kTileY;
-      const int h = ((tz + dz) * HY + (ty + dy)) * HX + (row + dx);
-      const v8bf av = load8(tile + (size_t)h * cin_p + ci);
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[nt], acc[mt][nt], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
-  {
-    const int r = wave * MT + mt, tz = r / kTileY, ty = r - tz * kTileY;
-    const int z = z0 + tz, y = y0 + ty;
-    if (z >= p.D || y >= p.H)
-      continue;
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-    {
-      const int co = nt * 16 + row;
-      if (co >= p.cout)
+      const int z = z0 + tz, y = y0 + ty;
+      if (z >= p.D || y >= p.H || x >= p.W)
         continue;
-      const float bv = p.bias[co];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-      {
-        const int x = x0 + kg * 4 + j;
-        if (x >= p.W)
-          continue;
-        float f = acc[mt][nt][j] + bv;
-        if (p.relu)
-          f = fmaxf(f, 0.f);
-        p.out[((size_t)(z * p.H + y) * p.W + x) * p.out_stride + co] = f2bf(f);
-      }
+      v4f f = acc[mt][nt] + bv;
+      if (p.relu)
+        f = __builtin_elementwise_max(f, v4f{0.f, 0.f, 0.f, 0.f});
+      const v2bf lo = __builtin_convertvector(v2f{f[0], f[1]}, v2bf);
+      const v2bf hi = __builtin_convertvector(v2f{f[2], f[3]}, v2bf);
+      v2u packed = {__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)};
+      *reinterpret_cast<v2u *>(p.out + ((size_t)(z * p.H + y) * p.W + x) * p.out_stride + co) =
+          packed;
     }
   }
 }
 
-// 2x2x2 average pooling, channels-last.
+// 2x2x2 average pooling, channels-last (C a multiple of 8): one thread per
+// output voxel and group of 8 channels, 16-byte loads and store.
 __global__ void
 pool_kernel(const uint16_t * in, uint16_t * out, int D, int H, int W, int C)
 {
-  const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1;
-  const size_t n = (size_t)D2 * H2 * W2 * C;
+  const int D2 = D >> 1, H2 = H >> 1, W2 = W >> 1, C8 = C >> 3;
+  const size_t n = (size_t)D2 * H2 * W2 * C8;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  const int c = (int)(i % C);
-  const size_t v = i / C;
+  const int c = (int)(i % C8) * 8;
+  const size_t v = i / C8;
   const int x = (int)(v % W2), y = (int)((v / W2) % H2), z = (int)(v / ((size_t)W2 * H2));
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int d = 0; d < 8; ++d)
   {
     const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
-    s += bf2f(in[((size_t)(zz * H + yy) * W + xx) * C + c]);
+    const v4u u = *reinterpret_cast<const v4u *>(in + ((size_t)(zz * H + yy) * W + xx) * C + c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      s[j] += bf2f((uint16_t)(u[j >> 1] >> (16 * (j & 1))));
   }
-  out[i] = f2bf(0.125f * s);
+  v4u o;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    o[j] = (uint32_t)f2bf(0.125f * s[2 * j]) | ((uint32_t)f2bf(0.125f * s[2 * j + 1]) << 16);
+  *reinterpret_cast<v4u *>(out + v * C + c) = o;
 }
 
 // Fixed-order partial sums of d^2 (RT threads, grid-stride).
@@ -368,12 +476,21 @@ features_f32_kernel(const float * in, int N, uint16_t * feat)
 }
 
 // 1x1x1 head: y = w . h + b.  Raw output for the stand-alone forward pass.
+// The voxel's channels (stride a multiple of 8) are read 8 at a time with
+// 16-byte loads; the sum runs over channels 0..C-1 in order.
 __device__ inline float
 head_value(const uint16_t * h, int stride, int C, const float * wb, int c)
 {
   float s = wb[C];
-  for (int k = 0; k < C; ++k)
-    s += wb[k] * bf2f(h[(size_t)c * stride + k]);
+  const uint16_t * row = h + (size_t)c * stride;
+  for (int k0 = 0; k0 < C; k0 += 8)
+  {
+    const v4u u = *reinterpret_cast<const v4u *>(row + k0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < C)
+        s += wb[k0 + j] * bf2f((uint16_t)(u[j >> 1] >> (16 * (j & 1))));
+  }
   return s;
 }
 
@@ -404,39 +521,55 @@ nblk(size_t n, int t)
   return (unsigned)((n + t - 1) / t);
 }
 
-// LDS bytes of conv3_tile_kernel's input halo.
-constexpr size_t tile_lds(int tz, int cin_p)
+template <int TZ>
+unsigned
+conv_tiles(const ConvArgs & p)
 {
-  return (size_t)(kTileX + 2) * (kTileY + 2) * (tz + 2) * cin_p * 2;
+  return (unsigned)(((p.W + kTileX - 1) / kTileX) * ((p.H + kTileY - 1) / kTileY) *
+                    ((p.D + TZ - 1) / TZ));
+}
+
+template <int NT, int TZ>
+int
+launch_tile(const ConvArgs & p, hipStream_t s)
+{
+  if (p.four)
+    hipLaunchKernelGGL((conv3_tile_kernel<NT, TZ, 4>), dim3(conv_tiles<TZ>(p)), dim3(RT), 0, s, p);
+  else
+    hipLaunchKernelGGL((conv3_tile_kernel<NT, TZ, 8>), dim3(conv_tiles<TZ>(p)), dim3(RT), 0, s, p);
+  return DOMA_OK;
 }
 
 template <int TZ>
 int
 launch_conv_tile(const ConvArgs & p, int cout16, hipStream_t s)
 {
-  const unsigned grid = (unsigned)(((p.W + kTileX - 1) / kTileX) *
-                                   ((p.H + kTileY - 1) / kTileY) * ((p.D + TZ - 1) / TZ));
-  const size_t lds = tile_lds(TZ, p.cin_p);
   if (cout16 == 16)
-    hipLaunchKernelGGL((conv3_tile_kernel<1, TZ>), dim3(grid), dim3(RT), lds, s, p);
-  else if (cout16 == 32)
-    hipLaunchKernelGGL((conv3_tile_kernel<2, TZ>), dim3(grid), dim3(RT), lds, s, p);
-  else if (cout16 == 64)
-    hipLaunchKernelGGL((conv3_tile_kernel<4, TZ>), dim3(grid), dim3(RT), lds, s, p);
-  else
-    return fail(DOMA_ERR_INVALID, "unsupported layer width");
-  return DOMA_OK;
+    return launch_tile<1, TZ>(p, s);
+  if (cout16 == 32)
+    return launch_tile<2, TZ>(p, s);
+  if (cout16 == 64)
+    return launch_tile<4, TZ>(p, s);
+  return fail(DOMA_ERR_INVALID, "unsupported layer width");
 }
 
 int
 launch_conv(const ConvArgs & p, int cout16, hipStream_t s)
 {
-  // The LDS-tiled kernel when tiles of 4 (or 2) z-planes do not straddle a
-  // sample and the halo fits 64 KiB of LDS; the direct kernel otherwise.
-  if (p.Ds % 4 == 0 && tile_lds(4, p.cin_p) <= (64u << 10))
-    return launch_conv_tile<4>(p, cout16, s);
-  if (p.Ds % 2 == 0 && tile_lds(2, p.cin_p) <= (64u << 10))
-    return launch_conv_tile<2>(p, cout16, s);
+  // The LDS-tiled kernel when the K layout is 27 taps of cin_p channels and the
+  // output channels (and the output row stride) come in fours, with tiles of 4,
+  // 2 or 1 z-planes: the deepest whose tiles do not straddle a sample and still
+  // number at least 256 (one per CU); the direct kernel otherwise.
+  const bool tiled = p.kpad >= 27 * p.cin_p && p.cout % 4 == 0 && p.out_stride % 4 == 0 &&
+                     (!p.four || (p.cin_p == 8 && p.cb == 0));
+  if (tiled)
+  {
+    if (p.Ds % 4 == 0 && conv_tiles<4>(p) >= 256)
+      return launch_conv_tile<4>(p, cout16, s);
+    if (p.Ds % 2 == 0 && conv_tiles<2>(p) >= 256)
+      return launch_conv_tile<2>(p, cout16, s);
+    return launch_conv_tile<1>(p, cout16, s);
+  }
   constexpr int MT = 2;
   const unsigned grid = nblk(p.V, (RT / 64) * 16 * MT);
   if (cout16 == 16)
@@ -452,9 +585,10 @@ launch_conv(const ConvArgs & p, int cout16, hipStream_t s)
 
 int
 conv(const PackedLayer & L, const uint16_t * a, const uint16_t * b, int D, int H, int W,
-     uint16_t * out, int out_stride, hipStream_t s, int Ds = 0)
+     uint16_t * out, int out_stride, hipStream_t s, int Ds = 0, bool four = false)
 {
   ConvArgs p{};
+  p.four = four;
   p.a = a;
   p.ca = L.cin_a_p;
   p.b = b;
@@ -478,7 +612,9 @@ conv(const PackedLayer & L, const uint16_t * a, const uint16_t * b, int D, int H
 int
 pool(const uint16_t * in, uint16_t * out, int D, int H, int W, int C, hipStream_t s)
 {
-  const size_t n = (size_t)(D / 2) * (H / 2) * (W / 2) * C;
+  if (C % 8)
+    return fail(DOMA_ERR_INVALID, "pooling needs a channel stride that is a multiple of 8");
+  const size_t n = (size_t)(D / 2) * (H / 2) * (W / 2) * (C / 8);
   hipLaunchKernelGGL(pool_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, D, H, W, C);
   return DOMA_OK;
 }
@@ -493,7 +629,8 @@ body(Net & n, hipStream_t s, int Ds = 0)
   const int s1 = pad8(C1), s2 = pad8(C2), s3 = pad8(C3);
   const int z1 = Ds > 0 ? Ds : D, z2 = z1 / 2, z4 = z1 / 4;
   int rc;
-  if ((rc = conv(n.layer[0], n.feat, nullptr, D, H, W, n.e1a, s1, s, z1)) ||
+  static_assert(CIN == 4, "layer 0 reads the 4 feature channels of an 8-channel stride");
+  if ((rc = conv(n.layer[0], n.feat, nullptr, D, H, W, n.e1a, s1, s, z1, true)) ||
       (rc = conv(n.layer[1], n.e1a, nullptr, D, H, W, n.e1b, s1, s, z1)) ||
       (rc = pool(n.e1b, n.p1, D, H, W, s1, s)) ||
       (rc = conv(n.layer[2], n.p1, nullptr, D2, H2, W2, n.e2a, s2, s, z2)) ||

@@ -31,10 +31,13 @@ def _torch_forward(flat, feats):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims", [(16, 20, 24), (32, 32, 32), (8, 12, 16)])
+@pytest.mark.parametrize("dims", [(16, 20, 24), (32, 32, 32), (8, 12, 16), (48, 40, 64),
+                                  (64, 64, 64)])
 def test_net_forward_matches_torch(dims):
     """BF16 activations and weights, FP32 accumulation: relative L2 error of the
-    output against FP32 PyTorch with the same (BF16-rounded) weights < 3e-2."""
+    output against FP32 PyTorch with the same (BF16-rounded) weights < 3e-2.
+    The grids take every convolution tiling: tiles of one z-plane on the small
+    grids, two at 48 x 40 x 64 and four at 64^3 (full resolution)."""
     torch = pytest.importorskip("torch")
     torch.manual_seed(1)
     flat = unet.bf16_round(unet.flatten(unet.make_model()))
