@@ -100,9 +100,10 @@ template <typename R>
 __device__ __forceinline__ R
 inv_weight(R n, R d)
 {
-  R r = fast_div(d, d - n);
+  const R dn = d - n;
+  R r = fast_div(d, dn);
   r = n < R(0) ? R(1) : r;
-  r = n >= R(0.5) * d ? R(2) : r;
+  r = dn <= n ? R(2) : r;  // n >= d/2
   return r;
 }
 
@@ -115,14 +116,16 @@ __device__ __forceinline__ void
 inv_weights3(double na, double da, bool va, double nb, double db, bool vb, double nc, double dc,
              bool vc, double & ia, double & ib, double & ic)
 {
-  // na < 0 is tested once, as the negation of na >= 0 (all operands are finite).
+  // na < 0 is tested once, as the negation of na >= 0 (all operands are finite);
+  // na < da/2 as da - na > na, on the difference the unclamped weight needs.
   const bool pa = na >= 0.0, pb = nb >= 0.0, pc = nc >= 0.0;
-  const bool ua = va && pa && na < 0.5 * da;
-  const bool ub = vb && pb && nb < 0.5 * db;
-  const bool uc = vc && pc && nc < 0.5 * dc;
-  const double Da = ua ? da - na : 1.0;
-  const double Db = ub ? db - nb : 1.0;
-  const double Dc = uc ? dc - nc : 1.0;
+  const double ea = da - na, eb = db - nb, ec = dc - nc;
+  const bool ua = va && pa && ea > na;
+  const bool ub = vb && pb && eb > nb;
+  const bool uc = vc && pc && ec > nc;
+  const double Da = ua ? ea : 1.0;
+  const double Db = ub ? eb : 1.0;
+  const double Dc = uc ? ec : 1.0;
   const double Dbc = Db * Dc, P = Da * Dbc;
   const double r = fast_rcp(P);
   const double rDa = r * Da;
@@ -139,13 +142,15 @@ template <typename R>
 __device__ __forceinline__ R
 twdd3d_fast(R qV, R sV, R mA, R eB, R xC, R & x, R & y, R & z)
 {
-  const R th = R(0.9);
-  const R qVs = qV * th;
+  // n_a = th qV + th (eB y + xC z) + mA x = T + (1 - th) mA x with
+  // T = th (qV + mA x + eB y + xC z), and likewise for b and c.
+  const R th = R(0.9), om = R(0.1);
   const R mAx = mA * x, eBy = eB * y, xCz = xC * z;
+  const R T = th * (qV + mAx + eBy + xCz);
   const R two_s = sV + R(2) * (mA + eB + xC);
-  const R na = qVs + (eBy + xCz) * th + mAx, da = (two_s - R(2) * mA) * x;
-  const R nb = qVs + (mAx + xCz) * th + eBy, db = (two_s - R(2) * eB) * y;
-  const R nc = qVs + (mAx + eBy) * th + xCz, dc = (two_s - R(2) * xC) * z;
+  const R na = om * mAx + T, da = (two_s - R(2) * mA) * x;
+  const R nb = om * eBy + T, db = (two_s - R(2) * eB) * y;
+  const R nc = om * xCz + T, dc = (two_s - R(2) * xC) * z;
   R ia, ib, ic;
   if constexpr (sizeof(R) == 8)
   {
