@@ -282,10 +282,10 @@ def learned_accel(n=128, groups=8, every=5, tol=1e-6, err_tol=5e-7, ref_tol=1e-1
                if params is not None else ""))
     if "learned_fp32" in out:
         t_l = out["learned_fp32"]["wall_s"]
-        best = min(out[k]["wall_s"] for k in ("plain_fp64", "plain_fp64_warm") if k in out)
+        plains = [out[k]["wall_s"] for k in ("plain_fp64", "plain_fp64_warm") if k in out]
         out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["wall_s"] / t_l, 3) \
             if "plain_fp64" in out else None
-        out["speedup_vs_best_plain"] = round(best / t_l, 3)
+        out["speedup_vs_best_plain"] = round(min(plains) / t_l, 3) if plains else None
         if "extrapolation_fp32" in out:
             out["speedup_vs_extrapolation"] = round(out["extrapolation_fp32"]["wall_s"] / t_l, 3)
         out["learned_max_err"] = max(out["learned_fp32"]["max_rel_err"])
@@ -366,8 +366,9 @@ def deck_mode(args):
 
 
 def learned_mode(args):
-    print(json.dumps(learned_accel(args.n, args.groups, args.every, weights_path=args.weights)),
-          flush=True)
+    runs = args.runs.split(",") if args.runs else None
+    print(json.dumps(learned_accel(args.n, args.groups, args.every, weights_path=args.weights,
+                                   runs=runs)), flush=True)
 
 
 def net_mode(args):
@@ -420,6 +421,9 @@ def main():
     ap.add_argument("--groups", type=int, default=8, help="groups (learned mode)")
     ap.add_argument("--every", type=int, default=5, help="correction interval (learned mode)")
     ap.add_argument("--weights", default=None, help="learned mode: correction weights (.npy)")
+    ap.add_argument("--runs", default=None,
+                    help="learned mode: comma-separated runs besides the FP64 reference "
+                         "(plain_fp64, plain_fp64_warm, extrapolation_fp32, learned_fp32)")
     ap.add_argument("--count", type=int, default=512, help="problems (batch mode)")
     ap.add_argument("--sample-cap", type=int, default=8, help="snapshots per problem and group")
     ap.add_argument("--workload", choices=["random_blocks", "multigroup_full", "high_scatter",
