@@ -202,7 +202,7 @@ def batch_mode(args):
 
 
 def learned_accel(n=128, groups=8, every=5, tol=1e-6, err_tol=5e-7, ref_tol=1e-10,
-                  weights_path=None, runs=None):
+                  weights_path=None, runs=None, cpu_threads=None):
     """Learned acceleration on the high-scattering workload (n^3 cells of 1 cm,
     `groups` groups, 256 directions, sigma_t log-uniform in [0.5, 10], c ~
     U[0.95, 0.995] with 98 % in-group, diamond difference).  Every run is timed
@@ -290,6 +290,50 @@ def learned_accel(n=128, groups=8, every=5, tol=1e-6, err_tol=5e-7, ref_tol=1e-1
             out["speedup_vs_extrapolation"] = round(out["extrapolation_fp32"]["wall_s"] / t_l, 3)
         out["learned_max_err"] = max(out["learned_fp32"]["max_rel_err"])
     out["tuning"] = {("fp32_sweep" if k == FP32_SWEEP else "fp64"): v for k, v in tunings.items()}
+    if cpu_threads:
+        out["cpu_reference"] = cpu_reference_sweeps(p, 8, 4, cpu_threads, out)
+    return out
+
+
+def cpu_reference_sweeps(problem, np_, na, threads, runs):
+    """The reference's own solver (oracle/_ref/ref_harness) on the same problem,
+    stopped after three source iterations of group 0 (fs_max_it = 3: the
+    reference then reports that SI did not converge and returns), timed inside
+    the harness; the time per sweep times each GPU run's sweep count is the
+    reference's time to the same answer, an extrapolation (a full CPU solve
+    would take hours).  None when the harness was not built."""
+    import re
+    import tempfile
+
+    from oracle import binding as ob
+
+    if not ob.REF_HARNESS.exists():
+        return None
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    log(f"bench: CPU reference sweeps on {problem.name}, {threads} threads")
+    with tempfile.TemporaryDirectory() as tmp:
+        _, _, logtxt = ob.run_reference(problem, np_, na, 1e-30, 3, 1e-30, 1, tmp,
+                                        nthreads=threads, tag="cpu_sweeps")
+    sweeps = len(re.findall(r"Performing SI", logtxt))
+    solve_s = float(re.findall(r"solve_seconds (\S+)", logtxt)[-1])
+    per = solve_s / max(sweeps, 1)
+    model, ncpu = host_cpu()
+    out = {"kind": "reference", "cores": threads, "host_cpu": model, "host_logical_cpus": ncpu,
+           "sample": f"the reference's own solveFixedSource ({'DD' if problem.scheme == 0 else 'TW-DD'}"
+                     f", compiled from its sources with -O2 -fopenmp) on {problem.name}: "
+                     f"{sweeps} sweeps of group 0 in {solve_s:.1f} s (solver set-up included) on "
+                     f"{threads} OpenMP threads",
+           "seconds_per_sweep": round(per, 3),
+           "grind_ns": round(per * 1e9 / (problem.N * problem.noct * problem.dirs), 4),
+           "extrapolated_time_to_solution_s": {}}
+    for k in ("plain_fp64", "plain_fp64_warm", "learned_fp32"):
+        if k in runs:
+            out["extrapolated_time_to_solution_s"][k] = round(per * runs[k]["sweeps"], 1)
+    if "plain_fp64" in runs and "learned_fp32" in runs:
+        out["learned_gpu_vs_reference_cpu"] = round(
+            per * runs["plain_fp64"]["sweeps"] / runs["learned_fp32"]["wall_s"], 1)
+    log(f"bench: CPU reference {per:.2f} s per sweep")
     return out
 
 
@@ -524,7 +568,8 @@ def main():
     ctx.close()
     learned = None
     if not args.no_learned:
-        learned = learned_accel(128, 8, 5)
+        learned = learned_accel(128, 8, 5, cpu_threads=None if args.no_cpu_baseline
+                                else min(os.cpu_count() or 1, 16))
     batched = None if args.no_batch else batch_run(512, 8)
     cpu = None
     if not args.no_cpu_baseline:
