@@ -85,6 +85,17 @@ def cpu_baseline_reference(problem, np_, na, tol, threads, gpu_phi=None, gpu_ms=
             float(x) for x in np.abs(gpu_phi - phi).max(axis=1) / np.abs(phi).max(axis=1)]
     if gpu_ms:
         out["gpu_speedup_time_to_solution"] = round(solve_s * 1e3 / gpu_ms, 1)
+    # The same solver on one thread, on a bounded sample: three source iterations
+    # of group 0 (the reference then reports non-convergence and returns).
+    with tempfile.TemporaryDirectory() as tmp:
+        _, _, log1 = ob.run_reference(problem, np_, na, 1e-30, 3, 1e-30, 1, tmp, nthreads=1,
+                                      tag="cpu_one_thread")
+    s1 = float(re.findall(r"solve_seconds (\S+)", log1)[-1])
+    n1 = len(re.findall(r"Performing SI", log1))
+    out["single_thread"] = {"value": round(s1 * 1e9 / (max(n1, 1) * units), 4),
+                            "unit": "ns/(cell*dir*grp)", "cores": 1,
+                            "sample": f"{n1} sweeps of group 0 in {s1:.2f} s on one thread "
+                                      f"(solver set-up included)"}
     return out
 
 
