@@ -146,10 +146,10 @@ shape_lds_bytes(int shape, int precision)
   const int nax = si.dims == 3 ? 3 : 2;
   const int nplanes = si.dims == 3 ? 3 * si.brick - 2 : 2 * si.brick - 1;
   // faces, cell records (48 B), plane table, plane offsets, inverse plane table,
-  // task slot
-  const size_t pi_off = (rs * fs * si.dl * si.ndl * nax * si.ko + 15) / 16 * 16 +
-                        48 * (size_t)nc * si.ko + 4 * (size_t)nc + 4 * (size_t)(nplanes + 1);
-  return (pi_off + 2 * (size_t)nc + 15) / 16 * 16 + 16;
+  // task slot -- the layout of sweep_impl.h
+  const size_t faces = rs * fs * si.dl * si.ndl * nax * si.ko;
+  const size_t ptb = sweepk::wide_plane_table(faces, nc, nplanes, si.ko) ? 8 : 4;
+  return sweepk::layout_bytes(faces, nc, nplanes, si.ko, ptb);
 }
 
 int

@@ -44,6 +44,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cellsolve.h"
 #include "internal.h"
 
@@ -159,6 +161,31 @@ struct alignas(16) CellRec
   int c, pad;
 };
 
+// LDS bytes of a brick layout with plane-table entries of `ptb` bytes.
+constexpr size_t
+layout_bytes(size_t face_bytes, int nc, int nplanes, int ko, size_t ptb)
+{
+  return ((face_bytes + 15) / 16 * 16 + 48 * (size_t)nc * ko + ptb * nc + 4 * (size_t)(nplanes + 1) +
+          2 * (size_t)nc + 15) / 16 * 16 + 16;
+}
+
+// Work-groups of `bytes` of LDS each that a CU's 160 KiB holds (allocated in
+// 512-byte granules).
+constexpr size_t
+lds_groups(size_t bytes)
+{
+  return (160 * 1024) / ((bytes + 511) / 512 * 512);
+}
+
+// Plane-table entries are 8 bytes (WIDE) unless that lowers the work-groups a
+// CU's LDS holds; then 4.
+constexpr bool
+wide_plane_table(size_t face_bytes, int nc, int nplanes, int ko)
+{
+  return lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 8)) ==
+         lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 4));
+}
+
 template <typename R, int DIM, int B, int DL, int NDL, int KO>
 struct Layout
 {
@@ -168,25 +195,37 @@ struct Layout
   static constexpr int NPLANES = DIM == 3 ? 3 * B - 2 : 2 * B - 1;
   static constexpr int NAX = DIM == 3 ? 3 : 2;
   static constexpr size_t FACE = sizeof(R) * FS * DW;      // one axis, one octant
+  static constexpr bool WIDE = wide_plane_table(FACE * NAX * KO, NC, NPLANES, KO);
+  static constexpr size_t PTB = WIDE ? 8 : 4;              // plane-table entry bytes
   static constexpr size_t FACES_OFF = 0;
   static constexpr size_t REC_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;   // CellRec [KO][NC]
-  static constexpr size_t PT_OFF = REC_OFF + sizeof(CellRec) * NC * KO;  // uint32 [NC]
-  static constexpr size_t PO_OFF = PT_OFF + 4 * NC;                        // int [NPLANES+1]
+  static constexpr size_t PT_OFF = REC_OFF + sizeof(CellRec) * NC * KO;  // entries [NC]
+  static constexpr size_t PO_OFF = PT_OFF + PTB * NC;                      // int [NPLANES+1]
   static constexpr size_t PI_OFF = PO_OFF + 4 * (NPLANES + 1);                // uint16 [NC]
   static constexpr size_t TASK_OFF = (PI_OFF + 2 * NC + 15) / 16 * 16;
   static constexpr size_t BYTES = TASK_OFF + 16;
+  static_assert(BYTES == layout_bytes(FACE * NAX * KO, NC, NPLANES, KO, PTB), "LDS layout");
 };
 
-// Plane table: per position of the plane-ordered cell list, the brick-local cell
-// ce and its x-, y- and z-face slots packed as ce | sx << 9 | sy << 15 | sz << 21
-// (3D: sx = lj B + lk, sy = li B + lk, sz = li B + lj; 2D: sx = lj, sy = li).
+// Plane table: per position of the plane-ordered cell list, where the cell's x-,
+// y- and z-face slots lie (3D: sx = lj B + lk, sy = li B + lk, sz = li B + lj;
+// 2D: sx = lj, sy = li; slot s of an axis starts s DW sizeof(R) bytes into it)
+// and the brick-local cell ce.  WIDE entries hold the byte offsets, as
+// {x | y << 16, z | ce << 16}, so a lane finds its faces with one add per axis;
+// narrow ones pack the slots as ce | sx << 9 | sy << 15 | sz << 21.
+typedef uint32_t PlaneEntry __attribute__((ext_vector_type(2)));
+
+template <bool WIDE>
+using PlaneEntryT = std::conditional_t<WIDE, PlaneEntry, uint32_t>;
+
 template <typename R, int DIM, int B, int DL, int NDL, int KO, int NT>
 __device__ __forceinline__ void
 load_plane_table(const BrickArgs & a, unsigned char * smem)
 {
   using L = Layout<R, DIM, B, DL, NDL, KO>;
-  static_assert(L::NC <= 512 && L::FS <= 64, "plane table packing");
-  uint32_t * pt = reinterpret_cast<uint32_t *>(smem + L::PT_OFF);
+  static_assert(L::NC <= 512 && L::FS <= 64 && (L::FS - 1) * L::DW * sizeof(R) < 65536,
+                "plane table packing");
+  PlaneEntryT<L::WIDE> * pt = reinterpret_cast<PlaneEntryT<L::WIDE> *>(smem + L::PT_OFF);
   int32_t * po = reinterpret_cast<int32_t *>(smem + L::PO_OFF);
   uint16_t * pi = reinterpret_cast<uint16_t *>(smem + L::PI_OFF);
   for (int e = threadIdx.x; e < L::NC; e += NT)
@@ -197,7 +236,13 @@ load_plane_table(const BrickArgs & a, unsigned char * smem)
     const uint32_t sx = DIM == 3 ? lj * B + lk : lj;
     const uint32_t sy = DIM == 3 ? li * B + lk : li;
     const uint32_t sz = DIM == 3 ? li * B + lj : 0;
-    pt[e] = ce | (sx << 9) | (sy << 15) | (sz << 21);
+    if constexpr (L::WIDE)
+    {
+      constexpr uint32_t step = L::DW * sizeof(R);
+      pt[e] = PlaneEntry{sx * step | (sy * step) << 16, sz * step | ce << 16};
+    }
+    else
+      pt[e] = ce | (sx << 9) | (sy << 15) | (sz << 21);
   }
   for (int e = threadIdx.x; e <= L::NPLANES; e += NT)
     po[e] = a.plane_off[e];
@@ -265,7 +310,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
 
   R * faces = reinterpret_cast<R *>(smem + L::FACES_OFF);  // [KO][NAX][FS][DW]
   CellRec * rec = reinterpret_cast<CellRec *>(smem + L::REC_OFF);
-  const uint32_t * pt = reinterpret_cast<const uint32_t *>(smem + L::PT_OFF);
+  using PE = PlaneEntryT<L::WIDE>;
+  const PE * pt = reinterpret_cast<const PE *>(smem + L::PT_OFF);
   const int32_t * po = reinterpret_cast<const int32_t *>(smem + L::PO_OFF);
   const uint16_t * pi = reinterpret_cast<const uint16_t *>(smem + L::PI_OFF);
 
@@ -470,10 +516,20 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
 
   // One cell solve of the lane: x-, y- and z-faces in LDS updated in place, the
   // direction-reduced scalar flux stored once per (cell, direction set).
-  auto solve = [&](uint32_t info, const CellRec & r) __attribute__((always_inline)) {
-    R * xf = fxo + ((info >> 9) & 63u) * DW;
-    R * yf = fyo + ((info >> 15) & 63u) * DW;
-    R * zf = fzo + ((info >> 21) & 63u) * DW;
+  auto solve = [&](PE info, const CellRec & r) __attribute__((always_inline)) {
+    R * xf, *yf, *zf;
+    if constexpr (L::WIDE)
+    {
+      xf = reinterpret_cast<R *>(reinterpret_cast<unsigned char *>(fxo) + (info.x & 0xFFFFu));
+      yf = reinterpret_cast<R *>(reinterpret_cast<unsigned char *>(fyo) + (info.x >> 16));
+      zf = reinterpret_cast<R *>(reinterpret_cast<unsigned char *>(fzo) + (info.y & 0xFFFFu));
+    }
+    else
+    {
+      xf = fxo + ((info >> 9) & 63u) * DW;
+      yf = fyo + ((info >> 15) & 63u) * DW;
+      zf = fzo + ((info >> 21) & 63u) * DW;
+    }
     double s = 0.0;
 #pragma unroll
     for (int m = 0; m < NDL; ++m)
@@ -541,7 +597,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   // brick outside the grid are solved too (on finite dummy data): they only
   // feed other outside cells, and their scalar flux is not stored.
   int pb = po[0] + cpo, pe = po[1];
-  uint32_t info_n = 0;
+  PE info_n{};
   CellRec r_n{};
   if (pb < pe)
   {
