@@ -482,9 +482,11 @@ brick_args(doma_ctx * c, const double * q)
 }
 
 // One sweep of group g over every octant: q = c->d_q, sigma_t = c->d_sig,
-// partials -> c->d_part.
+// partials -> c->d_part.  The persistent schedules time themselves through
+// c->sweep_ev (events on the kernel dispatch); run_sweep_with records those
+// events around the launches of the other schedules.
 int
-run_sweep_with(doma_ctx * c, const double * q)
+run_sweep_launch(doma_ctx * c, const double * q)
 {
   if (c->dims == 1)
   {
@@ -538,16 +540,15 @@ run_sweep_with(doma_ctx * c, const double * q)
     a.error = c->d_err;
     // 250 ms per dependency: orders of magnitude above any brick's latency.
     a.wait_ticks = (long long)c->wall_clock_khz * 250;
-    if (c->profiling)
-      DOMA_HIP(hipEventRecord(c->prof_events[0], c->stream));
+    // The launch's own dispatch records the timing events (kernel start and end).
+    a.ev_start = c->profiling ? c->prof_events[0] : c->sweep_ev[0];
+    a.ev_stop = c->profiling ? c->prof_events[1] : c->sweep_ev[1];
     int rc = launch_brick_sweep(a, c->scheme, c->precision, c->exact, c->shape, c->schedule,
                                 c->grid_flow, c->stream);
     if (rc)
       return rc;
     // Every work-group takes one ticket past the last task before it exits.
     c->ticket_base += (unsigned long long)a.ntasks + (unsigned long long)c->grid_flow;
-    if (c->profiling)
-      DOMA_HIP(hipEventRecord(c->prof_events[1], c->stream));
     return DOMA_OK;
   }
   const int ndiag = (int)c->diag_off.size() - 1;
@@ -564,6 +565,20 @@ run_sweep_with(doma_ctx * c, const double * q)
     if (c->profiling)
       DOMA_HIP(hipEventRecord(c->prof_events[2 * s + 1], c->stream));
   }
+  return DOMA_OK;
+}
+
+int
+run_sweep_with(doma_ctx * c, const double * q)
+{
+  const bool self_timed = c->dims >= 2 && (c->schedule == 1 || c->schedule == 3);
+  if (c->sweep_ev[0] && !self_timed)
+    DOMA_HIP(hipEventRecord(c->sweep_ev[0], c->stream));
+  int rc = run_sweep_launch(c, q);
+  if (rc)
+    return rc;
+  if (c->sweep_ev[1] && !self_timed)
+    DOMA_HIP(hipEventRecord(c->sweep_ev[1], c->stream));
   return DOMA_OK;
 }
 
@@ -649,10 +664,12 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
       IterArgs af = a;
       if (done_its + k == 0 && first_corr)
         af.corr = first_corr;  // warm start: the first increment is measured from prev_g
-      DOMA_HIP(hipEventRecord(c->si_events[2 * k], c->stream));
-      if ((rc = run_sweep(c, g)))
+      c->sweep_ev[0] = c->si_events[2 * k];
+      c->sweep_ev[1] = c->si_events[2 * k + 1];
+      rc = run_sweep(c, g);
+      c->sweep_ev[0] = c->sweep_ev[1] = nullptr;
+      if (rc)
         return rc;
-      DOMA_HIP(hipEventRecord(c->si_events[2 * k + 1], c->stream));
       if ((rc = enqueue_dsa(c, af)) || (rc = launch_finish(af, c->red_blocks, c->stream)) ||
           (rc = launch_residual_final(af, c->red_blocks, c->stream)))
         return rc;
@@ -736,11 +753,12 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       af.corr = c->d_corr;
     else if (it == 0 && first_corr)
       af.corr = first_corr;  // warm start: the first increment is measured from prev_g
-    DOMA_HIP(hipEventRecord(c->ev0, c->stream));
+    c->sweep_ev[0] = c->ev0;
+    c->sweep_ev[1] = c->ev1;
     int rc = run_sweep(c, g);
+    c->sweep_ev[0] = c->sweep_ev[1] = nullptr;
     if (rc)
       return rc;
-    DOMA_HIP(hipEventRecord(c->ev1, c->stream));
     if ((rc = enqueue_dsa(c, af)))
       return rc;
     if ((rc = launch_finish(af, c->red_blocks, c->stream)) ||
@@ -1837,19 +1855,27 @@ doma_profile_sweeps(doma_ctx * c, int32_t g, int32_t n, double * kernel_ms, doub
   return DOMA_OK;
 }
 
-// Autotuner objective: n sweeps each followed by the sum of their direction-set
-// partials (the read the finish kernel makes every iteration), so a shape that
-// writes many partial planes pays for them.  Leaves phi and q untouched.
+// Autotuner objective: n source iterations' worth of work -- each sweep of a
+// uniform positive source (scratch[2N..3N), so that every cell carries flux as
+// in a converging solve, whatever state the context is in) followed by the
+// finish kernel of an iteration (partials summed, residual terms, source
+// update) writing its flux and source into scratch[0..2N) -- so a shape pays
+// what an iteration pays for its partial planes.  Leaves phi and q untouched.
 int
-time_iterations(doma_ctx * c, int g, int n, double * ms_per_iteration)
+time_iterations(doma_ctx * c, int g, int n, double * ms_per_iteration, double * scratch)
 {
   int rc;
   IterArgs a = iter_args(c, g);
-  if ((rc = launch_gather_sig(a, c->stream)) || (rc = run_sweep(c, g)))
+  IterArgs f = a;
+  f.g = 0;  // the scratch flux holds one group
+  f.phi = scratch;
+  f.q = scratch + c->N;
+  const double * qs = scratch + 2 * (size_t)c->N;
+  if ((rc = launch_gather_sig(a, c->stream)) || (rc = run_sweep_with(c, qs)))
     return rc;  // warm-up
   DOMA_HIP(hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < n; ++i)
-    if ((rc = run_sweep(c, g)) || (rc = launch_reduce_parts(a, c->stream)))
+    if ((rc = run_sweep_with(c, qs)) || (rc = launch_finish(f, c->red_blocks, c->stream)))
       return rc;
   DOMA_HIP(hipEventRecord(c->ev1, c->stream));
   DOMA_HIP(hipEventSynchronize(c->ev1));
@@ -1896,27 +1922,67 @@ doma_autotune(doma_ctx * c, int32_t g, doma_tuning * best, double * best_ms)
     }
   }
   const doma_tuning keep = c->tuning;
-  doma_tuning win = keep;
-  double win_ms = 1e300;
-  for (const auto & t : cands)
+  // Screening: every candidate over 3 iterations.  Final: the best three again,
+  // twice each over 8 iterations in round-robin order, the fastest run winning
+  // (candidates within a few per cent of each other otherwise swap with noise).
+  std::vector<std::pair<double, doma_tuning>> timed;
+  double * scratch = nullptr;
+  if ((rc = dev_alloc(&scratch, 3 * (size_t)c->N)))
+    return rc;
+  struct Free
   {
+    double * p;
+    ~Free() { dev_free(p); }
+  } free_scratch{scratch};
+  if ((rc = launch_fill(scratch, 2 * (size_t)c->N, 0.0, c->stream)) ||
+      (rc = launch_fill(scratch + 2 * (size_t)c->N, c->N, 1.0, c->stream)))
+    return rc;
+  auto measure = [&](const doma_tuning & t, int n, double * ms) {
     c->tuning = t;
     if (configure_sweep(c) != DOMA_OK)
-      continue;  // shape not available for this direction count
-    double ms = 0.0;
-    if ((rc = time_iterations(c, g, 3, &ms)))
+      return 1;  // shape not available for this direction count
+    if ((rc = time_iterations(c, g, n, ms, scratch)))
     {
       c->tuning = keep;
       configure_sweep(c);
+      return -1;
+    }
+    return 0;
+  };
+  for (const auto & t : cands)
+  {
+    double ms = 0.0;
+    const int r = measure(t, 3, &ms);
+    if (r < 0)
       return rc;
-    }
-    if (ms < win_ms)
-    {
-      win_ms = ms;
-      win = t;
-      win.exact_arith = keep.exact_arith;
-    }
+    if (r == 0)
+      timed.emplace_back(ms, t);
   }
+  if (timed.empty())
+  {
+    c->tuning = keep;
+    configure_sweep(c);
+    return fail(DOMA_ERR_INVALID, "doma_autotune: no kernel shape fits this problem");
+  }
+  std::stable_sort(timed.begin(), timed.end(),
+                   [](const auto & x, const auto & y) { return x.first < y.first; });
+  const size_t finalists = std::min<size_t>(3, timed.size());
+  std::vector<double> best_of(finalists, 1e300);
+  for (int round = 0; round < 2 && finalists > 1; ++round)
+    for (size_t i = 0; i < finalists; ++i)
+    {
+      double ms = 0.0;
+      if (measure(timed[i].second, 8, &ms) < 0)
+        return rc;
+      best_of[i] = std::min(best_of[i], ms);
+    }
+  size_t pick = 0;
+  for (size_t i = 1; i < finalists; ++i)
+    if (best_of[i] < best_of[pick])
+      pick = i;
+  doma_tuning win = timed[pick].second;
+  win.exact_arith = keep.exact_arith;
+  const double win_ms = finalists > 1 ? best_of[pick] : timed[0].first;
   c->tuning = win;
   if ((rc = configure_sweep(c)))
     return rc;

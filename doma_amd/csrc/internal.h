@@ -76,6 +76,10 @@ struct BrickArgs
   int npencils;
   int * error;                      // set when a bounded wait expires
   long long wait_ticks;             // per-dependency deadline in device wall-clock ticks
+  // Host side only: events recorded by the persistent launch's own dispatch
+  // (its start and end), so that timing a sweep adds no marker to the stream.
+  hipEvent_t ev_start;
+  hipEvent_t ev_stop;
 
   // Device-side source iteration: when *skip != 0 (the iteration has converged)
   // the launch does nothing (null: always run).  A skipped dataflow launch still
@@ -356,6 +360,10 @@ struct doma_ctx
 
   // Per-launch HIP events (doma_profile_sweeps); recorded only while profiling.
   std::vector<hipEvent_t> prof_events;
+  // Start/end events of the next sweep (null: untimed).  The persistent
+  // schedules attach them to the kernel dispatch; the others record them around
+  // their launches.
+  hipEvent_t sweep_ev[2] = {nullptr, nullptr};
   bool profiling = false;
 
   // Dataflow schedule state.

@@ -42,6 +42,7 @@
 //     z, swept by one work-group that keeps the z faces in LDS.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -987,7 +988,10 @@ run_shape(const BrickArgs & a, int op, int schedule, int grid, hipStream_t s, in
       hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
       return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
     }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
+    if (a.ev_start || a.ev_stop)
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a.ev_start, a.ev_stop, 0, a);
+    else
+      hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
   }
   else if constexpr (DIM == 3)
   {
@@ -1001,7 +1005,10 @@ run_shape(const BrickArgs & a, int op, int schedule, int grid, hipStream_t s, in
       hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, S.NT, shmem);
       return e == hipSuccess ? DOMA_OK : hip_fail(e, "occupancy query");
     }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
+    if (a.ev_start || a.ev_stop)
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a.ev_start, a.ev_stop, 0, a);
+    else
+      hipLaunchKernelGGL(k, dim3(grid), dim3(S.NT), shmem, s, a);
   }
   else
     return fail(DOMA_ERR_INVALID, "the pencil schedule needs a 3D kernel shape");
