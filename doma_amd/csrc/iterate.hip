@@ -731,10 +731,14 @@ blocks_for(size_t n, int t)
 }
 } // namespace
 
+// Blocks of the fixed-order reductions over cells: one cell per thread up to
+// 2048 blocks (at 64^3, 1024 blocks keep 4 waves per SIMD of the finish
+// kernel's loads in flight; 256 blocks of 4 cells per thread left it latency
+// bound).
 int
 reduction_blocks(int N)
 {
-  int b = (N + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+  int b = (N + RED_THREADS - 1) / RED_THREADS;
   if (b < 1)
     b = 1;
   if (b > 2048)
