@@ -11,13 +11,21 @@ is absent, so the reference's 1D solver cannot run here).
   two neighbouring cells), so a dropped term, a wrong sign or a shifted index
   leaves a residual.  The moments are formed here from a textbook DD sweep in
   numpy, independent of the oracle.
-* The boundary rows (Larsen's conditions, :750-785) are restated as read; no
-  identity fixes them: parity unpinned for those two rows.  As built, the
+* The boundary rows (Larsen's conditions, :750-785) have no balance identity;
+  they are pinned against each other by mirror symmetry: the left row of a
+  slab read right to left is the right row of the original times -1 (the
+  reference writes the right row with the opposite sign), which a slip in one
+  row (the wrong half of the quadrature in beta, the wrong neighbour edge or
+  boundary cell, a sign of the diffusion or removal term) breaks.  What the two
+  rows share (the half-cell form and the source -q with no cell length) is
+  restated as read: parity unpinned for that part.  As built, the
   accelerated iteration converges in a few sweeps to a fixed point that differs
   from the unaccelerated one by a few percent (the boundary rows are not
   consistent with the transport boundary); that is asserted as measured.
 """
 from __future__ import annotations
+
+import dataclasses
 
 import numpy as np
 import pytest
@@ -132,3 +140,56 @@ def test_dsa_iteration_converges_fast_near_plain_si():
     assert st0["success"] and st1["success"]
     assert st1["si_iterations"][0] * 5 < st0["si_iterations"][0]
     assert np.linalg.norm(phi1 - phi0) < 0.05 * np.linalg.norm(phi0)
+
+
+def _mirror(p):
+    """The same slab read right to left (cells, materials and lengths reversed;
+    the quadrature is symmetric, so it is kept)."""
+    return dataclasses.replace(p, lx=p.lx[::-1].copy(), mat=p.mat[::-1].copy())
+
+
+@pytest.mark.parametrize("eigen", [0, 1])
+@pytest.mark.parametrize("seed", [0, 4])
+def test_dsa_boundary_rows_mirror(eigen, seed):
+    p = _slab(n=30, seed=seed, c=(0.8, 0.4), sig=(1.3, 3.1))
+    assert p.mat[0] != p.mat[-1] and p.lx[0] != p.lx[-1] and p.lx[-1] != p.lx[-2]
+    mu = p.mu.reshape(-1)
+    assert np.allclose(np.sort(mu), -np.sort(mu)[::-1])  # symmetric set
+    g = 0
+    rng = np.random.default_rng(seed + 10)
+    N = p.N
+    phi = rng.uniform(0.5, 2.0, N)
+    cur = rng.uniform(-0.3, 0.3, N)
+    edge = rng.uniform(0.5, 2.0, N + 1)  # one value per edge, shared by its two cells
+    sfr, sfl = edge[1:].copy(), edge[:-1].copy()
+    if eigen:  # keep the modified diffusion coefficients -h J / (sfR - sfL) positive
+        cur = -0.2 * (sfr - sfl)
+    qmg = 0.5 * p.src[p.mat, g]
+    lo, di, up, rhs = ob.dsa_system(p, g, eigen, phi, cur, sfr, sfl, qmg)
+    m = _mirror(p)
+    lo2, di2, up2, rhs2 = ob.dsa_system(m, g, eigen, phi[::-1].copy(), -cur[::-1], sfl[::-1].copy(),
+                                        sfr[::-1].copy(), qmg[::-1].copy())
+
+    def close(a, b):
+        return abs(a - b) <= 1e-13 * max(abs(a), abs(b), 1e-300)
+
+    # boundary rows: left row of the mirror = -(right row of the original)
+    assert close(di2[0], -di[N]) and close(up2[0], -lo[N]) and close(rhs2[0], -rhs[N])
+    assert close(di[0], -di2[N]) and close(up[0], -lo2[N]) and close(rhs[0], -rhs2[N])
+    # interior rows map onto each other with lo and up exchanged
+    k = np.arange(1, N)
+    assert np.allclose(di2[k], di[N - k], rtol=1e-13, atol=0)
+    assert np.allclose(lo2[k], up[N - k], rtol=1e-13, atol=0)
+    assert np.allclose(up2[k], lo[N - k], rtol=1e-13, atol=0)
+    assert np.allclose(rhs2[k], rhs[N - k], rtol=1e-12, atol=1e-15)
+    # planted slips in the oracle's right row (the same quadrature half as the
+    # left row's beta; the neighbour edge's coefficient from the next-to-last
+    # cell) break the identity
+    mu, w = p.mu.reshape(-1), p.w.reshape(-1)
+    beta = float(np.sum(np.abs(mu) * w)) / 2  # half-range current moment of the set
+    assert beta > 0
+    bad_di = di[N] + 2 * beta            # beta_right replaced by beta_left
+    assert not close(di2[0], -bad_di)
+    half_d = 0.5 / 3.0 / p.sigt[p.mat[-2], g] / p.lx[-2]
+    sr = 0.125 * (p.sigt[p.mat[-2], g] - p.sigs[p.mat[-2], g, g]) * p.lx[-2]
+    assert not close(up2[0], -(half_d - sr))
