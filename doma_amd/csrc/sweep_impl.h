@@ -125,8 +125,28 @@ constexpr int DPP_QP_XOR2 = 0x4E;
 constexpr int DPP_HALF_MIRROR = 0x141;
 constexpr int DPP_MIRROR = 0x140;
 
-// Fixed-order sum over groups of DL (4..64) consecutive lanes; every lane of a
-// group ends with the same bitwise result (each step adds a commutative pair).
+// Row broadcasts (GFX9 DPP): lane 15 of rows 0 and 2 to rows 1 and 3
+// (row_mask 0xA), lane 31 to rows 2 and 3 (row_mask 0xC).  Rows outside the
+// mask are not written and hold an unspecified value.
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double
+dpp_bcast(double v)
+{
+  const int lo = __double2loint(v);
+  const int hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, ROW_MASK, 0xF, false);
+  const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(rhi, rlo);
+}
+
+// Fixed-order sum over groups of DL (4..64) consecutive lanes.  The sum is
+// complete in the group's last lane (for DL <= 16 in every lane of the group);
+// each step adds a commutative pair, so the result is the same bitwise whichever
+// lane of a pair receives it.  Across the 16-lane rows the row sums move by
+// row broadcasts rather than a cross-lane permute through LDS.
 template <int DL>
 __device__ __forceinline__ double
 group_sum(double v)
@@ -139,9 +159,9 @@ group_sum(double v)
   if constexpr (DL >= 16)
     v += dpp_mov<DPP_MIRROR>(v);
   if constexpr (DL >= 32)
-    v += __shfl_xor(v, 16, 32);
+    v += dpp_bcast<DPP_ROW_BCAST15, 0xA>(v);  // rows 1, 3: r0 + r1, r2 + r3
   if constexpr (DL >= 64)
-    v += __shfl_xor(v, 32, 64);
+    v += dpp_bcast<DPP_ROW_BCAST31, 0xC>(v);  // row 3: (r2 + r3) + (r0 + r1)
   return v;
 }
 
@@ -586,7 +606,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
       s += w[m] * static_cast<double>(psi);
     }
     s = group_sum<DL>(s);
-    if (dd == 0 && r.c >= 0)
+    if (dd == DL - 1 && r.c >= 0)
       a.part[(size_t)ds * a.N + r.c] = s;
   };
 
