@@ -98,6 +98,7 @@ constexpr Shape kShapes[] = {
     {3, 4, 4, 1, 1, 64},    // 29: 4-direction sets, 4^3 bricks, one wave
     {3, 4, 4, 1, 2, 128},   // 30: 4-direction sets, 4^3 bricks, 2 octants
     {3, 4, 8, 1, 1, 128},   // 31: 8-direction sets, 4^3 bricks, 16 cells per pass
+    {3, 8, 8, 4, 1, 256},   // 32: <=32 dirs, 8^3 bricks, four directions per lane, 32 cells per pass
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 

@@ -124,6 +124,8 @@ SWEEP_SHAPES = [
     (29, 3, 17, 16, 19, 4, 4, capi.SCHEME_TW_DD),
     (30, 3, 13, 9, 11, 8, 4, capi.SCHEME_DD),
     (31, 3, 10, 9, 7, 4, 4, capi.SCHEME_TW_DD),
+    (32, 3, 11, 9, 10, 8, 4, capi.SCHEME_DD),
+    (32, 3, 13, 9, 11, 4, 3, capi.SCHEME_TW_DD),
 ]
 
 
