@@ -646,7 +646,8 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
   a.si_state = c->d_si_state;
   a.tol = tol;
   a.max_it = max_it;
-  if ((rc = launch_fill_i32(c->d_si_state, 4, 0, c->stream)))
+  if ((rc = launch_fill_i32(c->d_si_state, 4, 0, c->stream)) ||
+      (rc = launch_fill(c->d_res + 6, 2, 0.0, c->stream)))
     return rc;
   c->si_device = true;
   struct Off  // sweeps launched after the loop run unconditionally again
@@ -674,13 +675,15 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
           (rc = launch_residual_final(af, c->red_blocks, c->stream)))
         return rc;
     }
-    DOMA_HIP(hipMemcpyAsync(c->h_si_state, c->d_si_state, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                            c->stream));
-    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 4 * sizeof(double), hipMemcpyDeviceToHost,
+    // res[0..3] as si_loop reads them, res[6..7] = the loop state (the first
+    // iteration of every chunk runs, so its residual kernel has written them).
+    DOMA_HIP(hipMemcpyAsync(c->h_res, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
     DOMA_HIP(hipStreamSynchronize(c->stream));
     DOMA_HIP(hipGetLastError());
-    const int ran = c->h_si_state[1] - done_its;
+    const int state_done = c->h_res[6] != 0.0;
+    const int state_its = static_cast<int>(c->h_res[7]);
+    const int ran = state_its - done_its;
     for (int k = 0; k < ran && k < n; ++k)
     {
       float ms = 0.f;
@@ -690,10 +693,10 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
     }
     if (sweeps)
       *sweeps += ran;
-    done_its = c->h_si_state[1];
+    done_its = state_its;
     if (c->h_res[3] != 0.0)
       return reset_after_sweep_error(c);
-    done = c->h_si_state[0] != 0 || done_its >= max_it;
+    done = state_done || done_its >= max_it;
     chunk = std::min(2 * chunk, kMaxChunk);
   }
   if (its)
@@ -999,9 +1002,8 @@ create_ctx(const doma_problem_desc * d, hipStream_t shared, doma_ctx ** out)
       (rc = dev_alloc(&c->d_prev, N * c->G)))
     return cleanup(rc);
   c->red_blocks = reduction_blocks(c->N);
-  if ((rc = dev_alloc(&c->d_si_state, 4)) ||
-      hipHostMalloc(reinterpret_cast<void **>(&c->h_si_state), 4 * sizeof(int)) != hipSuccess)
-    return cleanup(rc ? rc : fail(DOMA_ERR_DEVICE, "hipHostMalloc failed"));
+  if ((rc = dev_alloc(&c->d_si_state, 4)))
+    return cleanup(rc);
   if ((rc = dev_alloc(&c->d_red, 2 * (size_t)c->red_blocks)) || (rc = dev_alloc(&c->d_res, 8)) ||
       (rc = dev_alloc(&c->d_err, 1)) ||
       (rc = dev_alloc(&c->d_ticket, (size_t)kTicketQueues * kTicketStride)))
@@ -1105,8 +1107,6 @@ doma_destroy(doma_ctx * c)
   dev_free(c->d_keep);
   if (c->h_res)
     (void)hipHostFree(c->h_res);
-  if (c->h_si_state)
-    (void)hipHostFree(c->h_si_state);
   dev_free(c->d_si_state);
   for (auto e : c->si_events)
     (void)hipEventDestroy(e);

@@ -189,7 +189,8 @@ struct IterArgs
   double * part;           // [ndirsets][N]
   double * dphi;           // [N] (used by the plain sweep entry point)
   double * red;            // reduction partials
-  double * res;            // [4] residual, diff, total, sweep error flag
+  double * res;            // [8] residual, diff, total, sweep error flag; [4..5] fission
+                           // rates (power iteration); [6..7] device SI loop state copy
   const int * err;         // sweep error flag (bounded wait expired)
   const double * corr;     // [N] correction applied before this sweep (or null)
   double * dcur;           // [N] receives this iteration's increment (or null)
@@ -379,7 +380,6 @@ struct doma_ctx
   int * d_err = nullptr;
   // Device-side source iteration: [0] done flag, [1] iterations (see IterArgs).
   int * d_si_state = nullptr;
-  int * h_si_state = nullptr;  // pinned copy
   bool si_device = false;      // sweeps launched now check d_si_state
   bool si_chunked = true;      // source iteration with the stopping rule on the device
   std::vector<hipEvent_t> si_events;

@@ -308,6 +308,10 @@ residual_body(const IterArgs & a, int nblocks)
       const int it = ++a.si_state[1];
       if (!(a.tol < r) || it >= a.max_it || a.res[3] != 0.0)
         a.si_state[0] = 1;
+      // A copy of the loop state beside the residual, so that the host reads
+      // both with one transfer.
+      a.res[6] = static_cast<double>(a.si_state[0]);
+      a.res[7] = static_cast<double>(it);
     }
   }
 }
