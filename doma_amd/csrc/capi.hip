@@ -752,6 +752,7 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     a.dcur = c->d_dcur;
   int it = 0;
   double r = 0.0, r_before = 0.0;
+  double first_ms = 0.0;  // duration of the loop's first sweep
   bool pending = false;
   // After a rejected correction the next attempt waits twice as long (up to 16x
   // `every`), so a network that does not help costs few wasted sweeps.
@@ -764,8 +765,13 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       af.corr = c->d_corr;
     else if (it == 0 && first_corr)
       af.corr = first_corr;  // warm start: the first increment is measured from prev_g
-    c->sweep_ev[0] = c->ev0;
-    c->sweep_ev[1] = c->ev1;
+    // The loop's first sweep is timed and the others are counted at its
+    // duration (timing events on a launch add idle gaps around it).
+    if (it == 0)
+    {
+      c->sweep_ev[0] = c->ev0;
+      c->sweep_ev[1] = c->ev1;
+    }
     int rc = run_sweep(c, g);
     c->sweep_ev[0] = c->sweep_ev[1] = nullptr;
     if (rc)
@@ -781,10 +787,14 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     DOMA_HIP(hipGetLastError());
     if (c->h_res[3] != 0.0)
       return reset_after_sweep_error(c);
-    float ms = 0.f;
-    DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (it == 0)
+    {
+      float ms = 0.f;
+      DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      first_ms = ms;
+    }
     if (sweep_ms)
-      *sweep_ms += ms;
+      *sweep_ms += first_ms;
     if (sweeps)
       ++*sweeps;
     const double rn = c->h_res[0];

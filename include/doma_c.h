@@ -103,10 +103,10 @@ typedef struct doma_stats {
   int32_t si_iterations[DOMA_MAX_GROUPS];  /* SI count per group, last MG pass */
   double si_residual[DOMA_MAX_GROUPS];
   double elapsed_ms;             /* wall time of the call on the context stream */
-  double sweep_ms;               /* time spent in sweep kernels (HIP events; the
-                                    device-side iteration loop times its first
-                                    sweep of each group and counts the group's
-                                    other sweeps at that duration) */
+  double sweep_ms;               /* time spent in sweep kernels (HIP events on
+                                    the first sweep of each group's source
+                                    iteration; its other sweeps are counted at
+                                    that duration) */
   int32_t corrections_accepted;  /* learned corrections kept (doma_set_learned_correction) */
   int32_t corrections_rejected;  /* learned corrections undone because the residual rose */
   double correction_ms;          /* wall time spent evaluating the correction network */
