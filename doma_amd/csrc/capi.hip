@@ -657,7 +657,7 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
   } off{c};
   int done_its = 0, chunk = 1;
   bool done = false;
-  double first_ms = 0.0;  // duration of the loop's first sweep
+  double timed_ms = 0.0;  // duration of the loop's last timed sweep
   while (!done)
   {
     const int n = std::min(chunk, max_it - done_its);
@@ -666,10 +666,11 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
       IterArgs af = a;
       if (done_its + k == 0 && first_corr)
         af.corr = first_corr;  // warm start: the first increment is measured from prev_g
-      // Only the loop's first sweep is timed: timing events on a launch cost
-      // idle gaps before and after it (about 11 us per sweep at 64^3), so the
-      // other sweeps are counted at its duration.
-      if (done_its + k == 0)
+      // Only the loop's first two sweeps are timed (the first of the first two
+      // chunks): timing events on a launch cost idle gaps before and after it
+      // (about 11 us per sweep at 64^3), so the later sweeps are counted at the
+      // second one's duration (the first often runs on cold caches).
+      if (k == 0 && done_its <= 1)
       {
         c->sweep_ev[0] = c->si_events[0];
         c->sweep_ev[1] = c->si_events[1];
@@ -691,14 +692,14 @@ si_loop_device(doma_ctx * c, int g, double tol, int max_it, int * its, double * 
     const int state_done = c->h_res[6] != 0.0;
     const int state_its = static_cast<int>(c->h_res[7]);
     const int ran = state_its - done_its;
-    if (done_its == 0 && ran > 0)
+    if (done_its <= 1 && ran > 0)
     {
       float ms = 0.f;
       DOMA_HIP(hipEventElapsedTime(&ms, c->si_events[0], c->si_events[1]));
-      first_ms = ms;
+      timed_ms = ms;
     }
     if (sweep_ms && ran > 0)
-      *sweep_ms += first_ms * std::min(ran, n);
+      *sweep_ms += timed_ms * std::min(ran, n);
     if (sweeps)
       *sweeps += ran;
     done_its = state_its;
@@ -752,7 +753,7 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     a.dcur = c->d_dcur;
   int it = 0;
   double r = 0.0, r_before = 0.0;
-  double first_ms = 0.0;  // duration of the loop's first sweep
+  double timed_ms = 0.0;  // duration of the loop's last timed sweep
   bool pending = false;
   // After a rejected correction the next attempt waits twice as long (up to 16x
   // `every`), so a network that does not help costs few wasted sweeps.
@@ -765,9 +766,10 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
       af.corr = c->d_corr;
     else if (it == 0 && first_corr)
       af.corr = first_corr;  // warm start: the first increment is measured from prev_g
-    // The loop's first sweep is timed and the others are counted at its
-    // duration (timing events on a launch add idle gaps around it).
-    if (it == 0)
+    // The loop's first two sweeps are timed and the later ones are counted at
+    // the second one's duration (timing events on a launch add idle gaps
+    // around it).
+    if (it <= 1)
     {
       c->sweep_ev[0] = c->ev0;
       c->sweep_ev[1] = c->ev1;
@@ -787,14 +789,14 @@ si_loop(doma_ctx * c, int g, double tol, int max_it, int * its, double * res, in
     DOMA_HIP(hipGetLastError());
     if (c->h_res[3] != 0.0)
       return reset_after_sweep_error(c);
-    if (it == 0)
+    if (it <= 1)
     {
       float ms = 0.f;
       DOMA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      first_ms = ms;
+      timed_ms = ms;
     }
     if (sweep_ms)
-      *sweep_ms += first_ms;
+      *sweep_ms += timed_ms;
     if (sweeps)
       ++*sweeps;
     const double rn = c->h_res[0];

@@ -104,9 +104,9 @@ typedef struct doma_stats {
   double si_residual[DOMA_MAX_GROUPS];
   double elapsed_ms;             /* wall time of the call on the context stream */
   double sweep_ms;               /* time spent in sweep kernels (HIP events on
-                                    the first sweep of each group's source
-                                    iteration; its other sweeps are counted at
-                                    that duration) */
+                                    the first two sweeps of each group's source
+                                    iteration; its later sweeps are counted at
+                                    the second one's duration) */
   int32_t corrections_accepted;  /* learned corrections kept (doma_set_learned_correction) */
   int32_t corrections_rejected;  /* learned corrections undone because the residual rose */
   double correction_ms;          /* wall time spent evaluating the correction network */
