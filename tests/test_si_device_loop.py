@@ -64,3 +64,21 @@ def test_dsa_eigen_transient_identical():
     a, b = _both(p, tr)
     for (ta, sa, fa, da), (tb, sb, fb, db) in zip(a, b):
         assert ta == tb and sa == sb and np.array_equal(fa, fb) and np.array_equal(da, db)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_sweep_time_statistic(device_loop):
+    """doma_stats.sweep_ms times the first two sweeps of each group's iteration
+    and counts the later ones at the second one's duration: per sweep it stays
+    within a small factor of the same kernel timed on its own."""
+    p = workloads.random_blocks(24, 2, 2, 2, block=8)
+    with doma_amd.Context(p, tuning=dict(schedule=1)) as ctx:
+        ctx.set_si_mode(device_loop)
+        _, st = ctx.solve_fixed_source(1e-8, 500, 1e-8, 20)
+        k_ms, _, _ = ctx.profile_sweeps(0, 5)  # kernel ms per sweep
+    assert st["success"] and st["sweeps"] > 10
+    per = st["sweep_ms"] / st["sweeps"]
+    print(f"device loop {device_loop}: {per:.4f} ms per sweep from the statistic, "
+          f"{k_ms:.4f} ms timed alone", flush=True)
+    assert 0.2 * k_ms < per < 5.0 * k_ms
