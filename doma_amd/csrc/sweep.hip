@@ -145,11 +145,13 @@ shape_lds_bytes(int shape, int precision)
   const int nc = si.dims == 3 ? si.brick * si.brick * si.brick : si.brick * si.brick;
   const int nax = si.dims == 3 ? 3 : 2;
   const int nplanes = si.dims == 3 ? 3 * si.brick - 2 : 2 * si.brick - 1;
-  // faces, cell records (48 B), plane table, plane offsets, inverse plane table,
-  // task slot -- the layout of sweep_impl.h
+  // faces, cell records (48 B in FP64, 24 B in FP32), plane table, plane
+  // offsets, inverse plane table, task slot -- the layout of sweep_impl.h
   const size_t faces = rs * fs * si.dl * si.ndl * nax * si.ko;
-  const size_t ptb = sweepk::wide_plane_table(faces, nc, nplanes, si.ko) ? 8 : 4;
-  return sweepk::layout_bytes(faces, nc, nplanes, si.ko, ptb);
+  const size_t rec = precision == DOMA_FP32_SWEEP ? sizeof(sweepk::CellRec<float>)
+                                                  : sizeof(sweepk::CellRec<double>);
+  const size_t ptb = sweepk::wide_plane_table(faces, nc, nplanes, si.ko, rec) ? 8 : 4;
+  return sweepk::layout_bytes(faces, nc, nplanes, si.ko, ptb, rec);
 }
 
 int

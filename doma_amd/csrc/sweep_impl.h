@@ -59,6 +59,7 @@ namespace sweepk
 struct Shape
 {
   int dims, B, DL, NDL, KO, NT;
+  int w32 = 0;  // FP32 kernels: waves per SIMD to budget registers for (0: compiler's choice)
 };
 
 // Shape table (index = shape id).  Chosen for direction chunks of <= 16, <= 32
@@ -98,7 +99,7 @@ constexpr Shape kShapes[] = {
     {3, 4, 4, 1, 1, 64},    // 29: 4-direction sets, 4^3 bricks, one wave
     {3, 4, 4, 1, 2, 128},   // 30: 4-direction sets, 4^3 bricks, 2 octants
     {3, 4, 8, 1, 1, 128},   // 31: 8-direction sets, 4^3 bricks, 16 cells per pass
-    {3, 8, 8, 4, 1, 256},   // 32: <=32 dirs, 8^3 bricks, four directions per lane, 32 cells per pass
+    {3, 8, 8, 4, 1, 256, 4}, // 32: <=32 dirs, 8^3 bricks, four directions per lane, 32 cells per pass
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
@@ -177,17 +178,22 @@ group_sum(double v)
 //   fast TW-DD 2D: v0 = q V, v1 = sigma V, g = (ly, lx, 1)
 //   fast DD:       v0 = q,   v1 = sigma,   g = (1/lx, 1/ly, 1/lz)
 //   exact:         v0 = q,   v1 = sigma,   g = (lx, ly, lz)
-struct alignas(16) CellRec
+// The fields have the sweep's arithmetic type (converted once, when the record
+// is built): 48 bytes in FP64, 24 in FP32.
+template <typename R>
+struct alignas(sizeof(R) * 2) CellRec
 {
-  double v0, v1, g0, g1, g2;
-  int c, pad;
+  R v0, v1, g0, g1, g2;
+  int c;
 };
+static_assert(sizeof(CellRec<double>) == 48 && sizeof(CellRec<float>) == 24, "cell records");
 
-// LDS bytes of a brick layout with plane-table entries of `ptb` bytes.
+// LDS bytes of a brick layout with records of `rec` bytes and plane-table
+// entries of `ptb` bytes.
 constexpr size_t
-layout_bytes(size_t face_bytes, int nc, int nplanes, int ko, size_t ptb)
+layout_bytes(size_t face_bytes, int nc, int nplanes, int ko, size_t ptb, size_t rec)
 {
-  return ((face_bytes + 15) / 16 * 16 + 48 * (size_t)nc * ko + ptb * nc + 4 * (size_t)(nplanes + 1) +
+  return ((face_bytes + 15) / 16 * 16 + rec * (size_t)nc * ko + ptb * nc + 4 * (size_t)(nplanes + 1) +
           2 * (size_t)nc + 15) / 16 * 16 + 16;
 }
 
@@ -202,10 +208,10 @@ lds_groups(size_t bytes)
 // Plane-table entries are 8 bytes (WIDE) unless that lowers the work-groups a
 // CU's LDS holds; then 4.
 constexpr bool
-wide_plane_table(size_t face_bytes, int nc, int nplanes, int ko)
+wide_plane_table(size_t face_bytes, int nc, int nplanes, int ko, size_t rec)
 {
-  return lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 8)) ==
-         lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 4));
+  return lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 8, rec)) ==
+         lds_groups(layout_bytes(face_bytes, nc, nplanes, ko, 4, rec));
 }
 
 template <typename R, int DIM, int B, int DL, int NDL, int KO>
@@ -217,17 +223,33 @@ struct Layout
   static constexpr int NPLANES = DIM == 3 ? 3 * B - 2 : 2 * B - 1;
   static constexpr int NAX = DIM == 3 ? 3 : 2;
   static constexpr size_t FACE = sizeof(R) * FS * DW;      // one axis, one octant
-  static constexpr bool WIDE = wide_plane_table(FACE * NAX * KO, NC, NPLANES, KO);
+  static constexpr size_t REC = sizeof(CellRec<R>);
+  static constexpr bool WIDE = wide_plane_table(FACE * NAX * KO, NC, NPLANES, KO, REC);
   static constexpr size_t PTB = WIDE ? 8 : 4;              // plane-table entry bytes
   static constexpr size_t FACES_OFF = 0;
   static constexpr size_t REC_OFF = ((FACE * NAX * KO) + 15) / 16 * 16;   // CellRec [KO][NC]
-  static constexpr size_t PT_OFF = REC_OFF + sizeof(CellRec) * NC * KO;  // entries [NC]
+  static constexpr size_t PT_OFF = REC_OFF + REC * NC * KO;               // entries [NC]
   static constexpr size_t PO_OFF = PT_OFF + PTB * NC;                      // int [NPLANES+1]
   static constexpr size_t PI_OFF = PO_OFF + 4 * (NPLANES + 1);                // uint16 [NC]
   static constexpr size_t TASK_OFF = (PI_OFF + 2 * NC + 15) / 16 * 16;
   static constexpr size_t BYTES = TASK_OFF + 16;
-  static_assert(BYTES == layout_bytes(FACE * NAX * KO, NC, NPLANES, KO, PTB), "LDS layout");
+  static_assert(BYTES == layout_bytes(FACE * NAX * KO, NC, NPLANES, KO, PTB, REC), "LDS layout");
 };
+
+// Register budget hint of a kernel: the shape's w32 waves per SIMD for FP32
+// sweeps whose work-group LDS leaves room for that many (so that registers do
+// not hold the work-groups per CU below what LDS allows), else none.  Shape 32
+// (the FP32 DD sweep's tuned shape) fits 4 work-groups of 4 waves per CU in LDS
+// and needs 131 VGPRs unhinted, 127 at a 4-wave budget, without spills.
+template <typename R, int DIM, int SHAPE>
+constexpr int
+waves_per_simd()
+{
+  constexpr Shape S = kShapes[SHAPE];
+  using L = Layout<R, DIM, S.B, S.DL, S.NDL, S.KO>;
+  constexpr int w = (int)lds_groups(L::BYTES) * (S.NT / 64) / 4;
+  return sizeof(R) == 4 && S.w32 > 1 && w >= S.w32 ? S.w32 : 1;
+}
 
 // Plane table: per position of the plane-ordered cell list, where the cell's x-,
 // y- and z-face slots lie (3D: sx = lj B + lk, sy = li B + lk, sz = li B + lj;
@@ -331,7 +353,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   static_assert(CPP % KO == 0, "cell slots must split evenly over octants");
 
   R * faces = reinterpret_cast<R *>(smem + L::FACES_OFF);  // [KO][NAX][FS][DW]
-  CellRec * rec = reinterpret_cast<CellRec *>(smem + L::REC_OFF);
+  CellRec<R> * rec = reinterpret_cast<CellRec<R> *>(smem + L::REC_OFF);
   using PE = PlaneEntryT<L::WIDE>;
   const PE * pt = reinterpret_cast<const PE *>(smem + L::PT_OFF);
   const int32_t * po = reinterpret_cast<const int32_t *>(smem + L::PO_OFF);
@@ -422,40 +444,43 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
       const int e = tid + (r0 + k) * NT;
       if (!(r0 + k < RI && e < NR))
         continue;
-      CellRec r;
-      r.v0 = 0.0;
-      r.v1 = 1.0;
-      r.g0 = r.g1 = r.g2 = 1.0;
-      r.c = cell[k];
-      r.pad = 0;
+      // The terms are formed in FP64 and rounded to R once, here.
+      double v0 = 0.0, v1 = 1.0, g0 = 1.0, g1 = 1.0, g2 = 1.0;
       if (cell[k] >= 0)
       {
         if (EXACT)
         {
-          r.v0 = q[k];
-          r.v1 = sg[k];
-          r.g0 = lx[k];
-          r.g1 = ly[k];
-          r.g2 = lz[k];
+          v0 = q[k];
+          v1 = sg[k];
+          g0 = lx[k];
+          g1 = ly[k];
+          g2 = lz[k];
         }
         else if (SCHEME == DOMA_SCHEME_DD)
         {
-          r.v0 = q[k];
-          r.v1 = sg[k];
-          r.g0 = 1.0 / lx[k];
-          r.g1 = 1.0 / ly[k];
-          r.g2 = 1.0 / lz[k];
+          v0 = q[k];
+          v1 = sg[k];
+          g0 = 1.0 / lx[k];
+          g1 = 1.0 / ly[k];
+          g2 = 1.0 / lz[k];
         }
         else
         {
           const double V = DIM == 3 ? lx[k] * ly[k] * lz[k] : lx[k] * ly[k];
-          r.v0 = q[k] * V;
-          r.v1 = sg[k] * V;
-          r.g0 = DIM == 3 ? ly[k] * lz[k] : ly[k];
-          r.g1 = DIM == 3 ? lx[k] * lz[k] : lx[k];
-          r.g2 = DIM == 3 ? lx[k] * ly[k] : 1.0;
+          v0 = q[k] * V;
+          v1 = sg[k] * V;
+          g0 = DIM == 3 ? ly[k] * lz[k] : ly[k];
+          g1 = DIM == 3 ? lx[k] * lz[k] : lx[k];
+          g2 = DIM == 3 ? lx[k] * ly[k] : 1.0;
         }
       }
+      CellRec<R> r;
+      r.v0 = R(v0);
+      r.v1 = R(v1);
+      r.g0 = R(g0);
+      r.g1 = R(g1);
+      r.g2 = R(g2);
+      r.c = cell[k];
       const int k_o = e / NC;
       rec[k_o * NC + pi[e - k_o * NC]] = r;
     }
@@ -534,11 +559,11 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   R * fxo = faces + (size_t)ko * NAX * FS * DW + dd * NDL;  // this lane's octant and directions
   R * fyo = fxo + FS * DW;
   R * fzo = fyo + FS * DW;
-  const CellRec * rco = rec + ko * NC;
+  const CellRec<R> * rco = rec + ko * NC;
 
   // One cell solve of the lane: x-, y- and z-faces in LDS updated in place, the
   // direction-reduced scalar flux stored once per (cell, direction set).
-  auto solve = [&](PE info, const CellRec & r) __attribute__((always_inline)) {
+  auto solve = [&](PE info, const CellRec<R> & r) __attribute__((always_inline)) {
     R * xf, *yf, *zf;
     if constexpr (L::WIDE)
     {
@@ -620,7 +645,7 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
   // feed other outside cells, and their scalar flux is not stored.
   int pb = po[0] + cpo, pe = po[1];
   PE info_n{};
-  CellRec r_n{};
+  CellRec<R> r_n{};
   if (pb < pe)
   {
     info_n = pt[pb];
@@ -670,7 +695,8 @@ sweep_brick(const BrickArgs & a, int bi, int bj, int bk, int og, int ch, unsigne
 
 // One launch per brick diagonal: block = (brick on the diagonal, octant group, chunk).
 template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
-__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+__global__ __launch_bounds__(kShapes[SHAPE].NT)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd<R, DIM, SHAPE>()))) void
 brick_sweep_kernel(BrickArgs a)
 {
   if (a.skip && *a.skip)
@@ -745,7 +771,8 @@ wait_flag(const BrickArgs & a, const uint32_t * f)
 // One persistent launch per sweep; tasks in ticket (= diagonal) order, task id
 // = (brick, octant group, chunk).
 template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
-__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+__global__ __launch_bounds__(kShapes[SHAPE].NT)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd<R, DIM, SHAPE>()))) void
 brick_flow_kernel(BrickArgs a)
 {
   if (a.skip && *a.skip)
@@ -823,7 +850,8 @@ brick_flow_kernel(BrickArgs a)
 // smaller tickets of the same queue, so the induction that rules out deadlock
 // there holds brick by brick.
 template <typename R, int DIM, int SCHEME, bool EXACT, int SHAPE>
-__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+__global__ __launch_bounds__(kShapes[SHAPE].NT)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd<R, DIM, SHAPE>()))) void
 brick_pencil_kernel(BrickArgs a)
 {
   if (a.skip && *a.skip)
@@ -891,7 +919,8 @@ brick_pencil_kernel(BrickArgs a)
 // across problems, interleaved by diagonal index).  Per-problem arguments are
 // read from a device table by the uniform problem index of the task.
 template <typename R, int SCHEME, int SHAPE>
-__global__ __launch_bounds__(kShapes[SHAPE].NT) void
+__global__ __launch_bounds__(kShapes[SHAPE].NT)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd<R, 3, SHAPE>()))) void
 brick_batch_kernel(BatchArgs b)
 {
   constexpr Shape S = kShapes[SHAPE];
